@@ -1,0 +1,129 @@
+/*
+ * se2w.h -- C ABI of libse2w_hip.so, the MI355X (gfx950) implementation of
+ * se2wave's explicit spectral-element time step.
+ *
+ * Plain C: pointers, sizes and int status codes only.  All arrays passed in
+ * are host arrays owned by the caller and copied during the call.  Nodal
+ * arrays are interleaved (x, y) per node in DMDA order, node = I + J*nx with
+ * nx = p*mx + 1 (se2wave.c:706-736); per-cell arrays are in element order
+ * e = ei + ej*mx.  Status: 0 = success, negative = error (like a
+ * PetscErrorCode); se2w_strerror() and se2w_last_error() describe it.
+ *
+ * Threading: one context per host thread; calls on a context are ordered on
+ * the context's own HIP stream; the se2w_get_* calls synchronise it.
+ */
+#ifndef SE2W_H
+#define SE2W_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Arithmetic modes. */
+#define SE2W_FP64 0       /* reference FP64 operation order, no FMA contraction */
+#define SE2W_FP32 1       /* FP32 operator and update, tables formed in FP64 */
+#define SE2W_FP32_KAHAN 2 /* FP32 with compensated displacement accumulation */
+
+/* Step schedules. */
+#define SE2W_SCHED_FUSED 0 /* one kernel per step: element operator + update */
+#define SE2W_SCHED_SPLIT 1 /* element kernel writes f, update kernel consumes it */
+
+/* Error codes. */
+#define SE2W_OK 0
+#define SE2W_ERR_ARG -1
+#define SE2W_ERR_MEM -2
+#define SE2W_ERR_HIP -3
+#define SE2W_ERR_STATE -4
+#define SE2W_ERR_UNSUPPORTED -5
+
+typedef struct se2w_ctx se2w_ctx;
+
+typedef struct {
+  int32_t mx, my;    /* elements in x and y */
+  int32_t p;         /* polynomial degree (1..8) */
+  int32_t precision; /* SE2W_FP64 | SE2W_FP32 | SE2W_FP32_KAHAN */
+  int32_t schedule;  /* SE2W_SCHED_FUSED | SE2W_SCHED_SPLIT */
+  int32_t device;    /* HIP device ordinal */
+} se2w_desc;
+
+/* Replaces SpecFECtxCreate + SpecFECtxCreateMesh_SEQ + DMCreateGlobalVector
+ * (se2wave.c:692-704, 825-884, 4594-4601): device storage for a mesh of
+ * mx*my elements of degree p; u = v = a = 0. */
+int se2w_create(const se2w_desc *desc, se2w_ctx **ctx);
+
+/* Replaces SpecFECtx's basis tables and per-element geometry
+ * (se2wave.c:860-868, 1238-1279): D[q*(p+1)+i] = dN_i/dxi at GLL node q,
+ * w1d[p+1] GLL weights, elem_dx[mx] / elem_dy[my] element widths
+ * (x of local node (p,0) minus x of local node (0,0), per element column;
+ * y of local node (p,p) minus y of local node (0,0), per element row). */
+int se2w_set_tables(se2w_ctx *ctx, const double *D, const double *w1d, const double *elem_dx,
+                    const double *elem_dy);
+
+/* Replaces SpecFECtx->cell_data (se2wave.c:29-33, 1148-1159): per-cell Lame
+ * parameters lambda[mx*my], mu[mx*my]. */
+int se2w_set_cells(se2w_ctx *ctx, const double *lambda, const double *mu);
+
+/* Replaces the Md vector of AssembleBilinearForm_Mass2d (se2wave.c:1431-1498):
+ * lumped mass, one value per node (the reference stores it on both dofs). */
+int se2w_set_mass(se2w_ctx *ctx, const double *mass);
+
+/* Replaces the PointwiseContext of SeismicSourceSetup_Pointwise
+ * (se2wave.c:3310-3365, 3367-3429): for each of ns sources the element index
+ * elem[s] and the 2*(p+1)^2 nodal values vals[s][2k+c] that
+ * SeismicSourceAddValues_Pointwise scales by the STF (se2wave.c:3277-3293). */
+int se2w_set_sources(se2w_ctx *ctx, int32_t ns, const int32_t *elem, const double *vals);
+
+/* Replaces the receiver location of RecordUVA_MultipleStations_NearestGLL_MPI
+ * (se2wave.c:2700-2805): element index and element-local GLL index of the
+ * node nearest to each of nr receivers. */
+int se2w_set_receivers(se2w_ctx *ctx, int32_t nr, const int32_t *elem, const int32_t *gll);
+
+/* Sets / reads the state vectors u, v, a (each 2*nx*ny doubles; NULL skips).
+ * Replaces VecGetArray access to the u, v, a Vecs of se2wave_demo. */
+int se2w_set_state(se2w_ctx *ctx, const double *u, const double *v, const double *a);
+int se2w_get_state(se2w_ctx *ctx, double *u, double *v, double *a);
+
+/* Replaces the time loop body of se2wave_demo (se2wave.c:4722-4775) for
+ * nsteps steps of size dt: predictor, SeismicSourceEvaluate with
+ * psi[k*ns + s] = stf_s(time_k) (3470-3497), AssembleLinearForm (1285-1429),
+ * f += g, a = f / M, corrector, receiver sampling (2904-2962).  The caller
+ * evaluates the source-time functions (its own STF objects) at the
+ * accumulated times. */
+int se2w_run(se2w_ctx *ctx, int32_t nsteps, double dt, const double *psi);
+
+/* Receiver rows of the last se2w_run: trace[(k*nr + r)*7 + c] with columns
+ * ux uy vx vy ax ay curl(v), as RecordUVA_MultipleStations_NearestGLL_MPI
+ * writes them (se2wave.c:2908-2959). */
+int se2w_get_receivers(se2w_ctx *ctx, double *trace);
+
+/* Replaces AssembleLinearForm_ElastoDynamics2d(c, u, F) (se2wave.c:1285-1429):
+ * f = -sum_e B^T C B u_e for the given u (2*nx*ny doubles).  The context's
+ * state is not touched. */
+int se2w_assemble_forces(se2w_ctx *ctx, const double *u, double *f);
+
+/* max, min and 2-norm over all dofs of u and of v, the quantities the
+ * reference prints every 100 steps (VecMax/VecMin/VecNorm, se2wave.c:4747-4755):
+ * out = {umax, umin, ul2, vmax, vmin, vl2}. */
+int se2w_norms(se2w_ctx *ctx, double out[6]);
+
+/* Wall time of the step loop of the last se2w_run (HIP events on the
+ * context's stream, predictor included) and, when per-launch profiling is
+ * on, the mean duration of the step kernel(s).  kernel_ms[0] = fused or
+ * element kernel, kernel_ms[1] = update kernel (split schedule). */
+int se2w_set_profiling(se2w_ctx *ctx, int32_t on);
+int se2w_get_timing(se2w_ctx *ctx, double *run_ms, double kernel_ms[2]);
+
+/* Tile shape (elements per workgroup) used by the step kernels. */
+int se2w_get_tile(se2w_ctx *ctx, int32_t *tex, int32_t *tey);
+
+int se2w_destroy(se2w_ctx *ctx);
+const char *se2w_strerror(int code);
+const char *se2w_last_error(void);
+int se2w_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SE2W_H */

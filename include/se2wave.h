@@ -1,0 +1,187 @@
+/*
+ * se2wave.h -- host-side C API of libse2w_host.so: the setup layer of
+ * se2wave (basis, mesh, materials, mass, time step, source-time functions,
+ * sources, receivers, options) and the solver that drives the GPU step
+ * through the C ABI of include/se2w.h.
+ *
+ * Names map one-to-one onto the reference's (se2wave.c of
+ * Nicolucas/se2dr_fork); each declaration cites the function it replaces.
+ * All arithmetic is FP64 and follows the reference's operation order, so the
+ * tables handed to the GPU are the reference's tables.
+ * Status codes: 0 = success, negative = error; se2w_host_last_error() holds
+ * the message (the analogue of SETERRQ's text).
+ */
+#ifndef SE2WAVE_H
+#define SE2WAVE_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#include "se2w.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SE2W_HOST_ERR_ARG -101
+#define SE2W_HOST_ERR_MEM -102
+#define SE2W_HOST_ERR_USER -103 /* PETSC_ERR_USER analogue */
+#define SE2W_HOST_ERR_SUP -104  /* PETSC_ERR_SUP analogue */
+#define SE2W_HOST_ERR_FILE -105
+
+const char *se2w_host_last_error(void);
+
+/* ---------------------------------------------------------------------- */
+/* Basis (se2wave.c:103-690)                                               */
+/* ---------------------------------------------------------------------- */
+/* CreateGLLCoordsWeights (103-209): xi ascending, w in the unflipped order. */
+int se2w_gll(int p, double *xi, double *w);
+/* TabulateBasis1d_CLEGENDRE (253-385): N[k*(p+1)+i] = N_i(x_k). */
+int se2w_basis1d(int p, int npoints, const double *x, double *N);
+/* TabulateBasisDerivatives1d_CLEGENDRE (387-523): dN[k*(p+1)+i] = N_i'(x_k). */
+int se2w_basis1d_deriv(int p, int npoints, const double *x, double *dN);
+/* Derivative matrix at the GLL nodes, D[q*(p+1)+i] (the table inside
+ * TabulateBasisDerivativesTensorProduct2d, 525-603). */
+int se2w_deriv_matrix(int p, double *D);
+
+/* ---------------------------------------------------------------------- */
+/* SpecFECtx (se2wave.c:35-54, 692-884, 1148-1236)                         */
+/* ---------------------------------------------------------------------- */
+typedef struct se2w_spec *SE2WSpec;
+
+/* SpecFECtxCreate + SpecFECtxCreateMesh_SEQ (692-704, 825-884): mesh of
+ * mx*my elements of degree p on [0,1]^2, DMDA node order. */
+int se2w_spec_create(int mx, int my, int p, SE2WSpec *spec);
+int se2w_spec_destroy(SE2WSpec *spec);
+/* SpecFECtxScaleMeshCoords (781-823): x = x*scale + shift per component. */
+int se2w_spec_scale_coords(SE2WSpec spec, const double scale[2], const double shift[2]);
+/* SpecFECtxSetConstantMaterialProperties(_Velocity) (1148-1175). */
+int se2w_spec_set_constant_material(SE2WSpec spec, double lambda, double mu, double rho);
+int se2w_spec_set_constant_material_velocity(SE2WSpec spec, double vp, double vs, double rho);
+/* Cell-wise material arrays (element order e = ei + ej*mx). */
+int se2w_spec_set_cell_material(SE2WSpec spec, const double *lambda, const double *mu, const double *rho);
+int se2w_spec_set_cell_material_velocity(SE2WSpec spec, const double *vp, const double *vs, const double *rho);
+
+typedef struct {
+  int32_t mx, my, p, nx, ny, npe;
+  int64_t nnodes;
+  double gmin[2], gmax[2]; /* DMGetBoundingBox */
+} se2w_spec_info;
+int se2w_spec_get_info(SE2WSpec spec, se2w_spec_info *info);
+/* Copies of internal arrays (caller-allocated). */
+int se2w_spec_get_coords(SE2WSpec spec, double *coords);      /* [2*nnodes] */
+int se2w_spec_get_material(SE2WSpec spec, double *lambda, double *mu, double *rho); /* [mx*my] each */
+int se2w_spec_get_tables(SE2WSpec spec, double *D, double *w1d, double *elem_dx, double *elem_dy);
+int se2w_spec_get_element(SE2WSpec spec, int32_t *element); /* [mx*my*npe] element -> node map */
+
+/* AssembleBilinearForm_Mass2d (1431-1498): M[node] (one value per node). */
+int se2w_spec_mass(SE2WSpec spec, double *mass);
+/* ElastoDynamicsComputeTimeStep_2d (2276-2349): CFL time step. */
+int se2w_spec_timestep(SE2WSpec spec, double *dt);
+
+/* ---------------------------------------------------------------------- */
+/* SeismicSTF (se2wave.c:59-64, 3431-3468, 3499-3822)                      */
+/* ---------------------------------------------------------------------- */
+typedef struct se2w_stf *SE2WSTF;
+struct se2w_stf {
+  char name[64];
+  void *data;
+  int (*evaluate)(SE2WSTF stf, double time, double *psi);
+  int (*destroy)(SE2WSTF stf);
+};
+int se2w_stf_create(const char *name, SE2WSTF *stf);                        /* SeismicSTFCreate 3431 */
+int se2w_stf_evaluate(SE2WSTF stf, double time, double *psi);                /* SeismicSTFEvaluate 3445 */
+int se2w_stf_destroy(SE2WSTF *stf);                                          /* SeismicSTFDestroy 3453 */
+int se2w_stf_create_ricker(double t0, double freq, double amp, SE2WSTF *stf); /* 3528-3545 */
+int se2w_stf_create_gaussian(double t0, double omega, double amp, SE2WSTF *stf); /* 3573-3590 */
+int se2w_stf_create_yoffe(double tau_s, double tau_r, double d_max, SE2WSTF *stf); /* 3795-3822 */
+
+/* ---------------------------------------------------------------------- */
+/* SeismicSource (se2wave.c:66-90, 2995-3429)                              */
+/* ---------------------------------------------------------------------- */
+#define SE2W_SOURCE_FORCE 0  /* SOURCE_TYPE_FORCE (rejected by the reference; point force here) */
+#define SE2W_SOURCE_MOMENT 1 /* SOURCE_TYPE_MOMENT */
+#define SE2W_SOURCE_IMPL_POINTWISE 0
+#define SE2W_SOURCE_IMPL_NEAREST_QPOINT 1
+#define SE2W_SOURCE_IMPL_DEFAULT 3
+
+typedef struct se2w_source *SE2WSource;
+/* SeismicSourceCreate (2995-3243): locate the element and the nearest
+ * (interior) GLL node.  values: force (Fx, Fy) or moment (Mxx, Mxy, Myx, Myy).
+ * use_nearest_internal_gll mirrors -use_nearest_internal_gll (default 1). */
+int se2w_source_create(SE2WSpec spec, int type, int impl, const double coor[2], const double *values,
+                       int use_nearest_internal_gll, SE2WSource *src);
+int se2w_source_setup(SE2WSource src);    /* SeismicSourceSetup 3245 */
+int se2w_source_destroy(SE2WSource *src); /* SeismicSourceDestroy 3259 */
+/* Element index, closest GLL node and the 2*npe nodal values. */
+int se2w_source_get(SE2WSource src, int32_t *elem, int32_t *closest_gll, double *vals);
+/* SeismicSourceEvaluate (3470-3497) on a host vector g[2*nnodes]. */
+int se2w_sources_evaluate(double time, int ns, SE2WSource *src, SE2WSTF *stf, double *g);
+
+/* ---------------------------------------------------------------------- */
+/* Receivers (RecordUVA_MultipleStations_NearestGLL, se2wave.c:2687-2993)   */
+/* ---------------------------------------------------------------------- */
+typedef struct se2w_receivers *SE2WReceivers;
+int se2w_receivers_create(SE2WSpec spec, int nr, const double *xr, SE2WReceivers *recv);
+int se2w_receivers_destroy(SE2WReceivers *recv);
+int se2w_receivers_get(SE2WReceivers recv, int32_t *nid, int32_t *elem, int32_t *gll);
+/* Writes the header of closestqpsource-receiverCP-uva-<mx>x<my>-p<p>-rank0.dat
+ * and the .mpimeta file into `dir` (2808-2877). */
+int se2w_receivers_write_header(SE2WReceivers recv, const char *dir);
+/* Appends rows: time[k] then, per receiver, the 7 values trace[(k*nr+r)*7+c]
+ * (2908-2962). */
+int se2w_receivers_append_rows(SE2WReceivers recv, int nsteps, const double *times, const double *trace);
+
+/* ---------------------------------------------------------------------- */
+/* Options database (PetscOptionsGet{Int,Real,Bool,RealArray} analogue)     */
+/* ---------------------------------------------------------------------- */
+typedef struct se2w_options *SE2WOptions;
+int se2w_options_create(int argc, char **argv, SE2WOptions *opts);
+int se2w_options_destroy(SE2WOptions *opts);
+int se2w_options_get_int(SE2WOptions opts, const char *name, int *value, int *found);
+int se2w_options_get_real(SE2WOptions opts, const char *name, double *value, int *found);
+int se2w_options_get_bool(SE2WOptions opts, const char *name, int *value, int *found);
+int se2w_options_get_string(SE2WOptions opts, const char *name, char *value, size_t len, int *found);
+/* Comma-separated list; *n is the capacity on input and the count on output. */
+int se2w_options_get_real_array(SE2WOptions opts, const char *name, double *values, int *n, int *found);
+
+/* ---------------------------------------------------------------------- */
+/* Seeded material models used by the benchmark workloads                   */
+/* ---------------------------------------------------------------------- */
+/* splitmix64 stream: uniform doubles in [0,1) with 53-bit resolution. */
+uint64_t se2w_rng_next(uint64_t *state);
+double se2w_rng_uniform(uint64_t *state);
+/* Per cell, element order: Vp ~ U[vp_lo,vp_hi], Vs = Vp/sqrt(3),
+ * rho ~ U[rho_lo,rho_hi] (draws vp then rho per cell). */
+int se2w_material_random_cells(int mx, int my, uint64_t seed, double vp_lo, double vp_hi, double rho_lo,
+                               double rho_hi, double *vp, double *vs, double *rho);
+/* nlayers horizontal layers of equal height; one (Vp, rho) draw per layer. */
+int se2w_material_layered(int mx, int my, int nlayers, uint64_t seed, double vp_lo, double vp_hi, double rho_lo,
+                          double rho_hi, double *vp, double *vs, double *rho);
+
+/* ---------------------------------------------------------------------- */
+/* Solver: the se2wave_demo time loop on the GPU (se2wave.c:4541-4807)      */
+/* ---------------------------------------------------------------------- */
+typedef struct se2w_solver *SE2WSolver;
+/* Builds the GPU context from the spec, sources (with their STFs) and
+ * receivers; the objects must outlive the solver. */
+int se2w_solver_create(SE2WSpec spec, int precision, int schedule, int ns, SE2WSource *src, SE2WSTF *stf,
+                       SE2WReceivers recv, SE2WSolver *solver);
+int se2w_solver_destroy(SE2WSolver *solver);
+se2w_ctx *se2w_solver_context(SE2WSolver solver);
+/* Advances nsteps steps of size dt from the solver's current time (time is
+ * accumulated as time = time + dt, se2wave.c:4724); receiver rows of these
+ * steps are available through se2w_solver_get_trace. */
+int se2w_solver_advance(SE2WSolver solver, int nsteps, double dt);
+double se2w_solver_time(SE2WSolver solver);
+int se2w_solver_steps_done(SE2WSolver solver);
+/* times[nsteps] and trace[nsteps][nr][7] of the last advance. */
+int se2w_solver_get_trace(SE2WSolver solver, double *times, double *trace);
+
+/* Runs the se2wave program (main + se2wave_demo) with reference options. */
+int se2w_main(int argc, char **argv);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SE2WAVE_H */

@@ -1,0 +1,495 @@
+"""se2dr_fork_amd -- MI355X-native explicit spectral-element time step of se2wave.
+
+Python mirror of the host C API (include/se2wave.h) and the GPU C ABI
+(include/se2w.h), loaded from the in-tree libraries lib/libse2w_host.so and
+lib/libse2w_hip.so.  The objects follow the reference's (se2wave.c of
+Nicolucas/se2dr_fork): SpecFECtx -> SpecFE, SeismicSTF -> STF,
+SeismicSource -> Source, RecordUVA_MultipleStations_NearestGLL -> Receivers,
+the se2wave_demo time loop -> Solver.  There is no fallback: if the native
+libraries are missing, importing the bindings raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional, Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+BIN_DIR = os.path.join(_HERE, "bin")
+
+FP64, FP32, FP32_KAHAN = 0, 1, 2
+SCHED_FUSED, SCHED_SPLIT = 0, 1
+SOURCE_FORCE, SOURCE_MOMENT = 0, 1
+IMPL_POINTWISE, IMPL_NEAREST_QPOINT, IMPL_DEFAULT = 0, 1, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32k": FP32_KAHAN}
+
+
+class Se2wError(RuntimeError):
+    pass
+
+
+_hip: Optional[C.CDLL] = None
+_host: Optional[C.CDLL] = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+
+
+def _ptr(a: np.ndarray, kind=_dp):
+    return a.ctypes.data_as(kind)
+
+
+def libraries():
+    """Load (once) and return (libse2w_hip, libse2w_host).  Raises if missing."""
+    global _hip, _host
+    if _hip is None:
+        hip_path = os.path.join(LIB_DIR, "libse2w_hip.so")
+        host_path = os.path.join(LIB_DIR, "libse2w_host.so")
+        for p in (hip_path, host_path):
+            if not os.path.exists(p):
+                raise Se2wError(f"native library {p} is missing: run __graft_entry__.build() or make -C se2dr_fork_amd")
+        _hip = C.CDLL(hip_path, mode=C.RTLD_GLOBAL)
+        _host = C.CDLL(host_path, mode=C.RTLD_GLOBAL)
+        _declare(_hip, _host)
+    return _hip, _host
+
+
+def _declare(hip, host):
+    v = C.c_void_p
+    i32 = C.c_int32
+    d = C.c_double
+    hip.se2w_strerror.restype = C.c_char_p
+    hip.se2w_last_error.restype = C.c_char_p
+    hip.se2w_create.argtypes = [v, C.POINTER(v)]
+    hip.se2w_set_tables.argtypes = [v, _dp, _dp, _dp, _dp]
+    hip.se2w_set_cells.argtypes = [v, _dp, _dp]
+    hip.se2w_set_mass.argtypes = [v, _dp]
+    hip.se2w_set_sources.argtypes = [v, i32, _ip, _dp]
+    hip.se2w_set_receivers.argtypes = [v, i32, _ip, _ip]
+    hip.se2w_set_state.argtypes = [v, _dp, _dp, _dp]
+    hip.se2w_get_state.argtypes = [v, _dp, _dp, _dp]
+    hip.se2w_run.argtypes = [v, i32, d, _dp]
+    hip.se2w_get_receivers.argtypes = [v, _dp]
+    hip.se2w_assemble_forces.argtypes = [v, _dp, _dp]
+    hip.se2w_norms.argtypes = [v, _dp]
+    hip.se2w_set_profiling.argtypes = [v, i32]
+    hip.se2w_get_timing.argtypes = [v, _dp, _dp]
+    hip.se2w_get_tile.argtypes = [v, _ip, _ip]
+    hip.se2w_destroy.argtypes = [v]
+    host.se2w_host_last_error.restype = C.c_char_p
+    host.se2w_gll.argtypes = [C.c_int, _dp, _dp]
+    host.se2w_basis1d.argtypes = [C.c_int, C.c_int, _dp, _dp]
+    host.se2w_basis1d_deriv.argtypes = [C.c_int, C.c_int, _dp, _dp]
+    host.se2w_deriv_matrix.argtypes = [C.c_int, _dp]
+    host.se2w_spec_create.argtypes = [C.c_int, C.c_int, C.c_int, C.POINTER(v)]
+    host.se2w_spec_destroy.argtypes = [C.POINTER(v)]
+    host.se2w_spec_scale_coords.argtypes = [v, _dp, _dp]
+    host.se2w_spec_set_constant_material.argtypes = [v, d, d, d]
+    host.se2w_spec_set_constant_material_velocity.argtypes = [v, d, d, d]
+    host.se2w_spec_set_cell_material.argtypes = [v, _dp, _dp, _dp]
+    host.se2w_spec_set_cell_material_velocity.argtypes = [v, _dp, _dp, _dp]
+    host.se2w_spec_get_info.argtypes = [v, v]
+    host.se2w_spec_get_coords.argtypes = [v, _dp]
+    host.se2w_spec_get_material.argtypes = [v, _dp, _dp, _dp]
+    host.se2w_spec_get_tables.argtypes = [v, _dp, _dp, _dp, _dp]
+    host.se2w_spec_get_element.argtypes = [v, _ip]
+    host.se2w_spec_mass.argtypes = [v, _dp]
+    host.se2w_spec_timestep.argtypes = [v, _dp]
+    host.se2w_stf_create_ricker.argtypes = [d, d, d, C.POINTER(v)]
+    host.se2w_stf_create_gaussian.argtypes = [d, d, d, C.POINTER(v)]
+    host.se2w_stf_create_yoffe.argtypes = [d, d, d, C.POINTER(v)]
+    host.se2w_stf_evaluate.argtypes = [v, d, _dp]
+    host.se2w_stf_destroy.argtypes = [C.POINTER(v)]
+    host.se2w_source_create.argtypes = [v, C.c_int, C.c_int, _dp, _dp, C.c_int, C.POINTER(v)]
+    host.se2w_source_setup.argtypes = [v]
+    host.se2w_source_destroy.argtypes = [C.POINTER(v)]
+    host.se2w_source_get.argtypes = [v, _ip, _ip, _dp]
+    host.se2w_sources_evaluate.argtypes = [d, C.c_int, C.POINTER(v), C.POINTER(v), _dp]
+    host.se2w_receivers_create.argtypes = [v, C.c_int, _dp, C.POINTER(v)]
+    host.se2w_receivers_destroy.argtypes = [C.POINTER(v)]
+    host.se2w_receivers_get.argtypes = [v, _ip, _ip, _ip]
+    host.se2w_receivers_write_header.argtypes = [v, C.c_char_p]
+    host.se2w_receivers_append_rows.argtypes = [v, C.c_int, _dp, _dp]
+    host.se2w_rng_uniform.argtypes = [C.POINTER(C.c_uint64)]
+    host.se2w_rng_uniform.restype = C.c_double
+    host.se2w_material_random_cells.argtypes = [C.c_int, C.c_int, C.c_uint64, d, d, d, d, _dp, _dp, _dp]
+    host.se2w_material_layered.argtypes = [C.c_int, C.c_int, C.c_int, C.c_uint64, d, d, d, d, _dp, _dp, _dp]
+    host.se2w_solver_create.argtypes = [v, C.c_int, C.c_int, C.c_int, C.POINTER(v), C.POINTER(v), v, C.POINTER(v)]
+    host.se2w_solver_destroy.argtypes = [C.POINTER(v)]
+    host.se2w_solver_context.argtypes = [v]
+    host.se2w_solver_context.restype = v
+    host.se2w_solver_advance.argtypes = [v, C.c_int, d]
+    host.se2w_solver_time.argtypes = [v]
+    host.se2w_solver_time.restype = d
+    host.se2w_solver_get_trace.argtypes = [v, _dp, _dp]
+    host.se2w_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+
+
+def _hchk(code):
+    if code != 0:
+        _, host = libraries()
+        raise Se2wError(f"se2wave host error {code}: {host.se2w_host_last_error().decode()}")
+
+
+def _gchk(code):
+    if code != 0:
+        hip, _ = libraries()
+        raise Se2wError(f"se2w error {code} ({hip.se2w_strerror(code).decode()}): {hip.se2w_last_error().decode()}")
+
+
+# ---------------------------------------------------------------------------
+# basis (se2wave.c:103-690)
+# ---------------------------------------------------------------------------
+def gll(p: int):
+    """CreateGLLCoordsWeights: ascending nodes, weights in the unflipped order."""
+    _, host = libraries()
+    xi = np.zeros(p + 1)
+    w = np.zeros(p + 1)
+    _hchk(host.se2w_gll(p, _ptr(xi), _ptr(w)))
+    return xi, w
+
+
+def deriv_matrix(p: int) -> np.ndarray:
+    _, host = libraries()
+    D = np.zeros((p + 1) * (p + 1))
+    _hchk(host.se2w_deriv_matrix(p, _ptr(D)))
+    return D.reshape(p + 1, p + 1)
+
+
+def basis1d(p: int, x, derivative=False) -> np.ndarray:
+    _, host = libraries()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    out = np.zeros(len(x) * (p + 1))
+    fn = host.se2w_basis1d_deriv if derivative else host.se2w_basis1d
+    _hchk(fn(p, len(x), _ptr(x), _ptr(out)))
+    return out.reshape(len(x), p + 1)
+
+
+class _SpecInfo(C.Structure):
+    _fields_ = [("mx", C.c_int32), ("my", C.c_int32), ("p", C.c_int32), ("nx", C.c_int32), ("ny", C.c_int32),
+                ("npe", C.c_int32), ("nnodes", C.c_int64), ("gmin", C.c_double * 2), ("gmax", C.c_double * 2)]
+
+
+class SpecFE:
+    """SpecFECtx: mesh of mx*my elements of degree p (DMDA node order)."""
+
+    def __init__(self, mx: int, my: int, p: int):
+        _, self._host = libraries()
+        self.h = C.c_void_p()
+        _hchk(self._host.se2w_spec_create(mx, my, p, C.byref(self.h)))
+        self.mx, self.my, self.p = mx, my, p
+        self.nx, self.ny = p * mx + 1, p * my + 1
+        self.npe = (p + 1) ** 2
+        self.nnodes = self.nx * self.ny
+
+    def close(self):
+        if self.h:
+            self._host.se2w_spec_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def scale_coords(self, scale, shift):
+        s = np.asarray(scale, dtype=np.float64)
+        t = np.asarray(shift, dtype=np.float64)
+        _hchk(self._host.se2w_spec_scale_coords(self.h, _ptr(s), _ptr(t)))
+
+    def set_constant_material_velocity(self, vp, vs, rho):
+        _hchk(self._host.se2w_spec_set_constant_material_velocity(self.h, vp, vs, rho))
+
+    def set_constant_material(self, lam, mu, rho):
+        _hchk(self._host.se2w_spec_set_constant_material(self.h, lam, mu, rho))
+
+    def set_cell_material_velocity(self, vp, vs, rho):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vp, vs, rho)]
+        _hchk(self._host.se2w_spec_set_cell_material_velocity(self.h, *[_ptr(x) for x in a]))
+
+    def set_cell_material(self, lam, mu, rho):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (lam, mu, rho)]
+        _hchk(self._host.se2w_spec_set_cell_material(self.h, *[_ptr(x) for x in a]))
+
+    def info(self):
+        inf = _SpecInfo()
+        _hchk(self._host.se2w_spec_get_info(self.h, C.byref(inf)))
+        return inf
+
+    def coords(self) -> np.ndarray:
+        c = np.zeros(2 * self.nnodes)
+        _hchk(self._host.se2w_spec_get_coords(self.h, _ptr(c)))
+        return c
+
+    def material(self):
+        ne = self.mx * self.my
+        lam, mu, rho = np.zeros(ne), np.zeros(ne), np.zeros(ne)
+        _hchk(self._host.se2w_spec_get_material(self.h, _ptr(lam), _ptr(mu), _ptr(rho)))
+        return lam, mu, rho
+
+    def tables(self):
+        nb = self.p + 1
+        D, w1d, dx, dy = np.zeros(nb * nb), np.zeros(nb), np.zeros(self.mx), np.zeros(self.my)
+        _hchk(self._host.se2w_spec_get_tables(self.h, _ptr(D), _ptr(w1d), _ptr(dx), _ptr(dy)))
+        return D, w1d, dx, dy
+
+    def element_map(self) -> np.ndarray:
+        el = np.zeros(self.mx * self.my * self.npe, dtype=np.int32)
+        _hchk(self._host.se2w_spec_get_element(self.h, _ptr(el, _ip)))
+        return el.reshape(self.mx * self.my, self.npe)
+
+    def mass(self) -> np.ndarray:
+        m = np.zeros(self.nnodes)
+        _hchk(self._host.se2w_spec_mass(self.h, _ptr(m)))
+        return m
+
+    def timestep(self) -> float:
+        dt = C.c_double()
+        _hchk(self._host.se2w_spec_timestep(self.h, C.byref(dt)))
+        return dt.value
+
+
+class STF:
+    """SeismicSTF object (function-pointer based; Ricker, Gaussian, Yoffe)."""
+
+    def __init__(self, kind: str, *params):
+        _, self._host = libraries()
+        self.h = C.c_void_p()
+        fn = {"ricker": self._host.se2w_stf_create_ricker, "gaussian": self._host.se2w_stf_create_gaussian,
+              "yoffe": self._host.se2w_stf_create_yoffe}[kind]
+        _hchk(fn(*[float(p) for p in params], C.byref(self.h)))
+        self.kind, self.params = kind, tuple(params)
+
+    def __call__(self, t: float) -> float:
+        v = C.c_double()
+        _hchk(self._host.se2w_stf_evaluate(self.h, t, C.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            self._host.se2w_stf_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Source:
+    """SeismicSource: created (located) and set up (nodal values)."""
+
+    def __init__(self, spec: SpecFE, kind: int, coor, values, impl=IMPL_NEAREST_QPOINT, use_nearest_internal_gll=True):
+        _, self._host = libraries()
+        self.spec = spec
+        self.h = C.c_void_p()
+        c = np.asarray(coor, dtype=np.float64)
+        v = np.zeros(4)
+        v[: len(values)] = values
+        _hchk(self._host.se2w_source_create(spec.h, kind, impl, _ptr(c), _ptr(v), int(bool(use_nearest_internal_gll)),
+                                            C.byref(self.h)))
+        _hchk(self._host.se2w_source_setup(self.h))
+
+    def get(self):
+        e, g = C.c_int32(), C.c_int32()
+        vals = np.zeros(2 * self.spec.npe)
+        _hchk(self._host.se2w_source_get(self.h, C.byref(e), C.byref(g), _ptr(vals)))
+        return e.value, g.value, vals
+
+    def close(self):
+        if self.h:
+            self._host.se2w_source_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Receivers:
+    """Nearest-GLL receivers and their output files."""
+
+    def __init__(self, spec: SpecFE, xr):
+        _, self._host = libraries()
+        self.spec = spec
+        self.xr = np.ascontiguousarray(np.asarray(xr, dtype=np.float64).reshape(-1))
+        self.nr = len(self.xr) // 2
+        self.h = C.c_void_p()
+        _hchk(self._host.se2w_receivers_create(spec.h, self.nr, _ptr(self.xr), C.byref(self.h)))
+
+    def nodes(self):
+        nid, el, g = (np.zeros(max(self.nr, 1), dtype=np.int32) for _ in range(3))
+        _hchk(self._host.se2w_receivers_get(self.h, _ptr(nid, _ip), _ptr(el, _ip), _ptr(g, _ip)))
+        return nid[: self.nr], el[: self.nr], g[: self.nr]
+
+    def write_header(self, directory: str):
+        _hchk(self._host.se2w_receivers_write_header(self.h, directory.encode()))
+
+    def append_rows(self, times, trace):
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        tr = np.ascontiguousarray(trace, dtype=np.float64)
+        _hchk(self._host.se2w_receivers_append_rows(self.h, len(t), _ptr(t), _ptr(tr)))
+
+    def close(self):
+        if self.h:
+            self._host.se2w_receivers_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Solver:
+    """se2wave_demo's time loop on the GPU (host-evaluated STFs, accumulated time)."""
+
+    def __init__(self, spec: SpecFE, sources: Sequence[Source], stfs: Sequence[STF], receivers: Optional[Receivers],
+                 precision=FP64, schedule=SCHED_FUSED):
+        _, self._host = libraries()
+        self.spec, self.sources, self.stfs, self.receivers = spec, list(sources), list(stfs), receivers
+        ns = len(self.sources)
+        srcs = (C.c_void_p * max(ns, 1))(*[s.h for s in self.sources])
+        stfs_ = (C.c_void_p * max(ns, 1))(*[s.h for s in self.stfs])
+        self.h = C.c_void_p()
+        _hchk(self._host.se2w_solver_create(spec.h, precision, schedule, ns, srcs, stfs_,
+                                            receivers.h if receivers else None, C.byref(self.h)))
+        self.ctx = Context(_handle=self._host.se2w_solver_context(self.h), spec=spec, precision=precision)
+        self.nr = receivers.nr if receivers else 0
+
+    def advance(self, nsteps: int, dt: float):
+        _hchk(self._host.se2w_solver_advance(self.h, nsteps, dt))
+        times = np.zeros(nsteps)
+        trace = np.zeros(nsteps * max(self.nr, 1) * 7)
+        _hchk(self._host.se2w_solver_get_trace(self.h, _ptr(times), _ptr(trace)))
+        return times, trace[: nsteps * self.nr * 7].reshape(nsteps, self.nr, 7)
+
+    @property
+    def time(self) -> float:
+        return self._host.se2w_solver_time(self.h)
+
+    def close(self):
+        if self.h:
+            self.ctx._handle = None
+            self._host.se2w_solver_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _Desc(C.Structure):
+    _fields_ = [("mx", C.c_int32), ("my", C.c_int32), ("p", C.c_int32), ("precision", C.c_int32),
+                ("schedule", C.c_int32), ("device", C.c_int32)]
+
+
+class Context:
+    """Direct handle on the GPU C ABI (include/se2w.h)."""
+
+    def __init__(self, mx=None, my=None, p=None, precision=FP64, schedule=SCHED_FUSED, device=0, _handle=None,
+                 spec: Optional[SpecFE] = None):
+        self._hip, _ = libraries()
+        self._owned = _handle is None
+        if spec is not None:
+            mx, my, p = spec.mx, spec.my, spec.p
+        self.mx, self.my, self.p, self.precision = mx, my, p, precision
+        self.nx, self.ny = p * mx + 1, p * my + 1
+        self.nnodes = self.nx * self.ny
+        self.npe = (p + 1) ** 2
+        if _handle is None:
+            h = C.c_void_p()
+            desc = _Desc(mx, my, p, precision, schedule, device)
+            _gchk(self._hip.se2w_create(C.byref(desc), C.byref(h)))
+            self._handle = h.value
+        else:
+            self._handle = _handle
+
+    def close(self):
+        if self._owned and self._handle:
+            self._hip.se2w_destroy(self._handle)
+        self._handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_tables(self, D, w1d, elem_dx, elem_dy):
+        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (D, w1d, elem_dx, elem_dy)]
+        _gchk(self._hip.se2w_set_tables(self._handle, *[_ptr(x) for x in a]))
+
+    def set_cells(self, lam, mu):
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (lam, mu)]
+        _gchk(self._hip.se2w_set_cells(self._handle, _ptr(a[0]), _ptr(a[1])))
+
+    def set_mass(self, m):
+        m = np.ascontiguousarray(m, dtype=np.float64)
+        _gchk(self._hip.se2w_set_mass(self._handle, _ptr(m)))
+
+    def set_sources(self, elem, vals):
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        v = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
+        _gchk(self._hip.se2w_set_sources(self._handle, len(e), _ptr(e, _ip), _ptr(v)))
+
+    def set_receivers(self, elem, gll_idx):
+        e = np.ascontiguousarray(elem, dtype=np.int32)
+        g = np.ascontiguousarray(gll_idx, dtype=np.int32)
+        _gchk(self._hip.se2w_set_receivers(self._handle, len(e), _ptr(e, _ip), _ptr(g, _ip)))
+
+    def set_state(self, u=None, v=None, a=None):
+        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, a)]
+        _gchk(self._hip.se2w_set_state(self._handle, *[None if x is None else _ptr(x) for x in arrs]))
+
+    def get_state(self):
+        u, v, a = (np.zeros(2 * self.nnodes) for _ in range(3))
+        _gchk(self._hip.se2w_get_state(self._handle, _ptr(u), _ptr(v), _ptr(a)))
+        return u, v, a
+
+    def run(self, nsteps: int, dt: float, psi=None):
+        p = None if psi is None else _ptr(np.ascontiguousarray(psi, dtype=np.float64))
+        _gchk(self._hip.se2w_run(self._handle, nsteps, dt, p))
+
+    def receivers(self, nsteps: int, nr: int) -> np.ndarray:
+        out = np.zeros(max(nsteps * nr * 7, 1))
+        _gchk(self._hip.se2w_get_receivers(self._handle, _ptr(out)))
+        return out[: nsteps * nr * 7].reshape(nsteps, nr, 7)
+
+    def assemble_forces(self, u) -> np.ndarray:
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        f = np.zeros_like(u)
+        _gchk(self._hip.se2w_assemble_forces(self._handle, _ptr(u), _ptr(f)))
+        return f
+
+    def norms(self) -> np.ndarray:
+        out = np.zeros(6)
+        _gchk(self._hip.se2w_norms(self._handle, _ptr(out)))
+        return out
+
+    def set_profiling(self, on: bool):
+        _gchk(self._hip.se2w_set_profiling(self._handle, int(on)))
+
+    def timing(self):
+        ms = C.c_double()
+        k = np.zeros(2)
+        _gchk(self._hip.se2w_get_timing(self._handle, C.byref(ms), _ptr(k)))
+        return ms.value, k
+
+    def tile(self):
+        a, b = C.c_int32(), C.c_int32()
+        _gchk(self._hip.se2w_get_tile(self._handle, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+
+def run_program(args: Sequence[str]) -> int:
+    """Runs the se2wave program (se2w_main) in-process with reference options."""
+    _, host = libraries()
+    argv = [b"se2wave"] + [a.encode() for a in args]
+    arr = (C.c_char_p * len(argv))(*argv)
+    return host.se2w_main(len(argv), arr)

@@ -1,0 +1,789 @@
+// se2w_capi.hip -- host side of libse2w_hip.so: the C ABI declared in
+// include/se2w.h, device memory management and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "se2w.h"
+#include "se2w_kernels.hpp"
+
+using namespace se2w;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+#define HIPCHK(call)                                                                         \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess) return fail(SE2W_ERR_HIP, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+// Conversion between the caller's DMDA double layout and the padded device
+// layout of the context's precision.
+template <typename R>
+__global__ void k_from_dmda(const double *src, typename V2<R>::T *dst, int nx, int ny, int pitch) {
+  const long n = (long)nx * ny;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long J = i / nx, I = i - J * nx;
+    dst[J * pitch + I] = typename V2<R>::T{(R)src[2 * i], (R)src[2 * i + 1]};
+  }
+}
+
+template <typename R>
+__global__ void k_to_dmda(const typename V2<R>::T *src, const typename V2<R>::T *comp, double *dst, int nx, int ny,
+                          int pitch) {
+  const long n = (long)nx * ny;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long J = i / nx, I = i - J * nx;
+    const typename V2<R>::T s = src[J * pitch + I];
+    double x = (double)s.x, y = (double)s.y;
+    if (comp) {  // compensated value: s - c
+      const typename V2<R>::T c = comp[J * pitch + I];
+      x -= (double)c.x;
+      y -= (double)c.y;
+    }
+    dst[2 * i] = x;
+    dst[2 * i + 1] = y;
+  }
+}
+
+template <typename R>
+__global__ void k_scalar_from_dmda(const double *src, R *dst, int nx, int ny, int pitch, int invert) {
+  const long n = (long)nx * ny;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long J = i / nx, I = i - J * nx;
+    const double m = src[i];
+    dst[J * pitch + I] = invert ? (R)(1.0 / m) : (R)m;
+  }
+}
+
+// Partial max/min/sum-of-squares of u and v over blocks (double accumulation).
+template <typename R>
+__global__ __launch_bounds__(256) void k_norms(const typename V2<R>::T *u, const typename V2<R>::T *v, int nx, int ny,
+                                               int pitch, double *partial) {
+  __shared__ double red[6][256];
+  double umax = -1.0e300, umin = 1.0e300, usq = 0.0, vmax = -1.0e300, vmin = 1.0e300, vsq = 0.0;
+  const long n = (long)nx * ny;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) {
+    const long J = i / nx, I = i - J * nx;
+    const typename V2<R>::T a = u[J * pitch + I], b = v[J * pitch + I];
+    const double ax = a.x, ay = a.y, bx = b.x, by = b.y;
+    umax = fmax(umax, fmax(ax, ay));
+    umin = fmin(umin, fmin(ax, ay));
+    usq += ax * ax + ay * ay;
+    vmax = fmax(vmax, fmax(bx, by));
+    vmin = fmin(vmin, fmin(bx, by));
+    vsq += bx * bx + by * by;
+  }
+  const int t = threadIdx.x;
+  red[0][t] = umax; red[1][t] = umin; red[2][t] = usq;
+  red[3][t] = vmax; red[4][t] = vmin; red[5][t] = vsq;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (t < s) {
+      red[0][t] = fmax(red[0][t], red[0][t + s]);
+      red[1][t] = fmin(red[1][t], red[1][t + s]);
+      red[2][t] += red[2][t + s];
+      red[3][t] = fmax(red[3][t], red[3][t + s]);
+      red[4][t] = fmin(red[4][t], red[4][t + s]);
+      red[5][t] += red[5][t + s];
+    }
+    __syncthreads();
+  }
+  if (t == 0)
+    for (int c = 0; c < 6; c++) partial[blockIdx.x * 6 + c] = red[c][0];
+}
+
+size_t real_size(int mode) { return mode == SE2W_FP64 ? sizeof(double) : sizeof(float); }
+
+}  // namespace
+
+struct se2w_ctx {
+  int mx = 0, my = 0, p = 0, nb = 0, npe = 0, nx = 0, ny = 0, pitch = 0;
+  int mode = 0, schedule = 0, device = 0;
+  int tex = 0, tey = 0, tiles_x = 0, tiles_y = 0, ntiles = 0, tiles_per_xcd = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  void *U[2] = {nullptr, nullptr}, *C[2] = {nullptr, nullptr};
+  void *V = nullptr, *A = nullptr, *F = nullptr, *Utmp = nullptr;
+  void *mass = nullptr, *lambda = nullptr, *mu = nullptr, *edx = nullptr, *edy = nullptr, *D = nullptr,
+       *w2 = nullptr;
+  int cur = 0;
+  bool have_tables = false, have_cells = false, have_mass = false;
+  // host copies needed for receiver post-processing
+  std::vector<double> hD, hdx, hdy;
+  // sources
+  int ns = 0;
+  int *d_src_elem = nullptr, *d_tsrc_ptr = nullptr, *d_tsrc_idx = nullptr;
+  void *d_src_vals = nullptr;
+  void *d_psi = nullptr;
+  size_t psi_cap = 0;
+  // receivers
+  int nr = 0;
+  std::vector<int> h_rec_elem, h_rec_gll;
+  int *d_rec_elem = nullptr, *d_trec_ptr = nullptr, *d_trec_idx = nullptr;
+  void *d_rec = nullptr;
+  size_t rec_cap = 0;
+  int rec_steps = 0;
+  std::vector<double> trace;
+  // timing
+  bool profiling = false;
+  double last_ms = 0.0, kern_ms[2] = {0.0, 0.0};
+  double *d_partial = nullptr;
+};
+
+namespace {
+
+int dev_alloc(void **p, size_t bytes) {
+  if (*p) return SE2W_OK;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) {
+    *p = nullptr;
+    return fail(SE2W_ERR_MEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  }
+  return SE2W_OK;
+}
+
+void dev_free(void *&p) {
+  if (p) hipFree(p);
+  p = nullptr;
+}
+
+template <typename T>
+int upload(void **dst, const T *src, size_t n, hipStream_t s) {
+  dev_free(*dst);
+  int ierr = dev_alloc(dst, sizeof(T) * std::max<size_t>(n, 1));
+  if (ierr) return ierr;
+  if (n) HIPCHK(hipMemcpyAsync(*dst, src, sizeof(T) * n, hipMemcpyHostToDevice, s));
+  HIPCHK(hipStreamSynchronize(s));
+  return SE2W_OK;
+}
+
+// Real-typed upload of a double host array.
+int upload_real(se2w_ctx *c, void **dst, const double *src, size_t n) {
+  if (c->mode == SE2W_FP64) return upload<double>(dst, src, n, c->stream);
+  std::vector<float> tmp(n);
+  for (size_t i = 0; i < n; i++) tmp[i] = (float)src[i];
+  return upload<float>(dst, tmp.data(), n, c->stream);
+}
+
+// Tile shapes of the kernels compiled for (p, precision).
+void tile_shape(int p, int mode, int &tex, int &tey) {
+#define CASE(P_)                                                                                     \
+  case P_:                                                                                           \
+    if (mode == SE2W_FP64) { tex = TileShape<P_, double>::TEX; tey = TileShape<P_, double>::TEY; } \
+    else { tex = TileShape<P_, float>::TEX; tey = TileShape<P_, float>::TEY; }                      \
+    break;
+  switch (p) { CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) default: tex = tey = 0; }
+#undef CASE
+}
+
+// Per-tile CSR of the items (sources or receivers) whose element touches
+// nodes owned by the tile.
+void build_tile_lists(const se2w_ctx *c, const std::vector<int> &elem, std::vector<int> &ptr, std::vector<int> &idx) {
+  const int P = c->p;
+  std::vector<std::vector<int>> lists(c->ntiles);
+  for (size_t s = 0; s < elem.size(); s++) {
+    const int ex = elem[s] % c->mx, ey = elem[s] / c->mx;
+    const int x0 = ex * P, x1 = ex * P + P, y0 = ey * P, y1 = ey * P + P;  // node block, inclusive
+    for (int ty = 0; ty < c->tiles_y; ty++) {
+      const int ey0 = ty * c->tey, ey_end = std::min(ey0 + c->tey, c->my);
+      const int oy0 = ey0 * P, oy1 = ey_end * P - 1 + (ey_end == c->my ? 1 : 0);
+      if (y1 < oy0 || y0 > oy1) continue;
+      for (int tx = 0; tx < c->tiles_x; tx++) {
+        const int ex0 = tx * c->tex, ex_end = std::min(ex0 + c->tex, c->mx);
+        const int ox0 = ex0 * P, ox1 = ex_end * P - 1 + (ex_end == c->mx ? 1 : 0);
+        if (x1 < ox0 || x0 > ox1) continue;
+        lists[tx + ty * c->tiles_x].push_back((int)s);
+      }
+    }
+  }
+  ptr.assign(c->ntiles + 1, 0);
+  idx.clear();
+  for (int t = 0; t < c->ntiles; t++) {
+    ptr[t + 1] = ptr[t] + (int)lists[t].size();
+    idx.insert(idx.end(), lists[t].begin(), lists[t].end());
+  }
+}
+
+template <int MODE>
+StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double dt, int last) {
+  using R = typename RealOf<MODE>::T;
+  using R2 = typename V2<R>::T;
+  StepParams<R> p;
+  memset(&p, 0, sizeof(p));
+  p.mx = c->mx; p.my = c->my; p.nx = c->nx; p.ny = c->ny; p.pitch = c->pitch;
+  p.tiles_x = c->tiles_x; p.tiles_y = c->tiles_y; p.tiles_per_xcd = c->tiles_per_xcd;
+  p.u_in = (const R2 *)c->U[c->cur];
+  p.u_out = (R2 *)c->U[c->cur ^ 1];
+  p.v = (R2 *)c->V;
+  p.a = (R2 *)c->A;
+  p.f = (R2 *)c->F;
+  p.c_in = (const R2 *)c->C[c->cur];
+  p.c_out = (R2 *)c->C[c->cur ^ 1];
+  p.mass = (const R *)c->mass;
+  p.lambda = (const R *)c->lambda;
+  p.mu = (const R *)c->mu;
+  p.elem_dx = (const R *)c->edx;
+  p.elem_dy = (const R *)c->edy;
+  p.D = (const R *)c->D;
+  p.w2 = (const R *)c->w2;
+  p.tile_src_ptr = c->d_tsrc_ptr; p.tile_src_idx = c->d_tsrc_idx; p.src_elem = c->d_src_elem;
+  p.src_vals = (const R *)c->d_src_vals; p.psi = (const R *)c->d_psi; p.ns = c->ns;
+  p.tile_rec_ptr = c->d_trec_ptr; p.tile_rec_idx = c->d_trec_idx; p.rec_elem = c->d_rec_elem;
+  p.rec = (R *)c->d_rec; p.nr = c->nr;
+  p.step = step;
+  // dt, 0.5*dt and 0.5*dt*dt are formed in FP64 as in se2wave.c:4726-4745
+  p.dt = (R)dt;
+  p.hdt = (R)(0.5 * dt);
+  p.hdt2 = (R)(0.5 * dt * dt);
+  p.last = last;
+  return p;
+}
+
+template <int P, int MODE>
+void launch_step_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm, hipEvent_t *evs) {
+  const dim3 grid(8 * c->tiles_per_xcd), block(kThreads);
+  if (evs) hipEventRecord(evs[0], c->stream);
+  if (c->schedule == SE2W_SCHED_FUSED) {
+    hipLaunchKernelGGL((tile_kernel<P, MODE, true>), grid, block, 0, c->stream, prm);
+    if (evs) hipEventRecord(evs[1], c->stream);
+  } else {
+    hipLaunchKernelGGL((tile_kernel<P, MODE, false>), grid, block, 0, c->stream, prm);
+    if (evs) hipEventRecord(evs[1], c->stream);
+    hipLaunchKernelGGL((update_kernel<P, MODE>), grid, block, 0, c->stream, prm);
+    if (evs) hipEventRecord(evs[2], c->stream);
+  }
+}
+
+template <int MODE>
+void launch_step(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm, hipEvent_t *evs) {
+  switch (c->p) {
+    case 1: launch_step_p<1, MODE>(c, prm, evs); break;
+    case 2: launch_step_p<2, MODE>(c, prm, evs); break;
+    case 3: launch_step_p<3, MODE>(c, prm, evs); break;
+    case 4: launch_step_p<4, MODE>(c, prm, evs); break;
+    case 5: launch_step_p<5, MODE>(c, prm, evs); break;
+    case 6: launch_step_p<6, MODE>(c, prm, evs); break;
+    case 7: launch_step_p<7, MODE>(c, prm, evs); break;
+    case 8: launch_step_p<8, MODE>(c, prm, evs); break;
+    default: break;
+  }
+}
+
+template <int P, int MODE>
+void launch_element_only_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
+  hipLaunchKernelGGL((tile_kernel<P, MODE, false>), dim3(8 * c->tiles_per_xcd), dim3(kThreads), 0, c->stream, prm);
+}
+
+template <int MODE>
+void launch_element_only(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
+  switch (c->p) {
+    case 1: launch_element_only_p<1, MODE>(c, prm); break;
+    case 2: launch_element_only_p<2, MODE>(c, prm); break;
+    case 3: launch_element_only_p<3, MODE>(c, prm); break;
+    case 4: launch_element_only_p<4, MODE>(c, prm); break;
+    case 5: launch_element_only_p<5, MODE>(c, prm); break;
+    case 6: launch_element_only_p<6, MODE>(c, prm); break;
+    case 7: launch_element_only_p<7, MODE>(c, prm); break;
+    case 8: launch_element_only_p<8, MODE>(c, prm); break;
+    default: break;
+  }
+}
+
+int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 4096); }
+
+template <int MODE>
+int run_mode(se2w_ctx *c, int nsteps, double dt) {
+  using R = typename RealOf<MODE>::T;
+  std::vector<hipEvent_t> evs;
+  if (c->profiling) {
+    evs.resize((size_t)nsteps * 3);
+    for (auto &e : evs) HIPCHK(hipEventCreate(&e));
+  }
+  HIPCHK(hipEventRecord(c->ev0, c->stream));
+  {
+    StepParams<R> prm = make_params<MODE>(c, 0, dt, 0);
+    const long n = (long)c->pitch * c->ny;
+    hipLaunchKernelGGL((predictor_kernel<MODE>), dim3(grid_for(n)), dim3(kThreads), 0, c->stream, prm);
+    HIPCHK(hipGetLastError());
+    c->cur ^= 1;
+  }
+  for (int k = 0; k < nsteps; k++) {
+    StepParams<R> prm = make_params<MODE>(c, k, dt, k == nsteps - 1);
+    launch_step<MODE>(c, prm, c->profiling ? &evs[(size_t)k * 3] : nullptr);
+    c->cur ^= 1;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventRecord(c->ev1, c->stream));
+  HIPCHK(hipEventSynchronize(c->ev1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+  c->last_ms = ms;
+  c->kern_ms[0] = c->kern_ms[1] = 0.0;
+  if (c->profiling) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < nsteps; k++) {
+      float t0 = 0.f, t1 = 0.f;
+      hipEventElapsedTime(&t0, evs[(size_t)k * 3], evs[(size_t)k * 3 + 1]);
+      s0 += t0;
+      if (c->schedule == SE2W_SCHED_SPLIT) {
+        hipEventElapsedTime(&t1, evs[(size_t)k * 3 + 1], evs[(size_t)k * 3 + 2]);
+        s1 += t1;
+      }
+    }
+    c->kern_ms[0] = s0 / nsteps;
+    c->kern_ms[1] = s1 / nsteps;
+    for (auto &e : evs) hipEventDestroy(e);
+  }
+  return SE2W_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int se2w_version(void) { return 1; }
+
+const char *se2w_last_error(void) { return g_err; }
+
+const char *se2w_strerror(int code) {
+  switch (code) {
+    case SE2W_OK: return "success";
+    case SE2W_ERR_ARG: return "invalid argument";
+    case SE2W_ERR_MEM: return "out of device memory";
+    case SE2W_ERR_HIP: return "HIP runtime error";
+    case SE2W_ERR_STATE: return "context not fully set up";
+    case SE2W_ERR_UNSUPPORTED: return "unsupported configuration";
+    default: return "unknown error";
+  }
+}
+
+int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
+  if (!desc || !out) return fail(SE2W_ERR_ARG, "null argument");
+  *out = nullptr;
+  if (desc->mx < 1 || desc->my < 1) return fail(SE2W_ERR_ARG, "mx, my must be >= 1");
+  if (desc->p < 1 || desc->p > kMaxP) return fail(SE2W_ERR_UNSUPPORTED, "p = %d outside 1..%d", desc->p, kMaxP);
+  if (desc->precision < SE2W_FP64 || desc->precision > SE2W_FP32_KAHAN)
+    return fail(SE2W_ERR_ARG, "unknown precision %d", desc->precision);
+  if (desc->schedule != SE2W_SCHED_FUSED && desc->schedule != SE2W_SCHED_SPLIT)
+    return fail(SE2W_ERR_ARG, "unknown schedule %d", desc->schedule);
+  const long nx = (long)desc->p * desc->mx + 1, ny = (long)desc->p * desc->my + 1;
+  if (nx * ny * 2 > 0x7fffffffL) return fail(SE2W_ERR_UNSUPPORTED, "mesh too large for 32-bit dof indices");
+  se2w_ctx *c = new se2w_ctx();
+  c->mx = desc->mx; c->my = desc->my; c->p = desc->p; c->nb = desc->p + 1; c->npe = c->nb * c->nb;
+  c->nx = (int)nx; c->ny = (int)ny;
+  c->pitch = (int)((nx + 31) / 32 * 32);
+  c->mode = desc->precision;
+  c->schedule = desc->schedule;
+  c->device = desc->device;
+  tile_shape(c->p, c->mode, c->tex, c->tey);
+  c->tiles_x = (c->mx + c->tex - 1) / c->tex;
+  c->tiles_y = (c->my + c->tey - 1) / c->tey;
+  c->ntiles = c->tiles_x * c->tiles_y;
+  c->tiles_per_xcd = (c->ntiles + 7) / 8;
+  int ierr = SE2W_OK;
+  hipError_t he = hipSetDevice(c->device);
+  if (he != hipSuccess) { ierr = fail(SE2W_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); goto bad; }
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    ierr = fail(SE2W_ERR_HIP, "stream/event creation failed");
+    goto bad;
+  }
+  {
+    const size_t rs = real_size(c->mode);
+    const size_t field = (size_t)c->pitch * c->ny * 2 * rs;
+    if ((ierr = dev_alloc(&c->U[0], field)) || (ierr = dev_alloc(&c->U[1], field)) || (ierr = dev_alloc(&c->V, field)) ||
+        (ierr = dev_alloc(&c->A, field)))
+      goto bad;
+    if (c->mode == SE2W_FP32_KAHAN) {
+      if ((ierr = dev_alloc(&c->C[0], field)) || (ierr = dev_alloc(&c->C[1], field))) goto bad;
+      hipMemsetAsync(c->C[0], 0, field, c->stream);
+      hipMemsetAsync(c->C[1], 0, field, c->stream);
+    }
+    if (c->schedule == SE2W_SCHED_SPLIT) {
+      if ((ierr = dev_alloc(&c->F, field))) goto bad;
+      hipMemsetAsync(c->F, 0, field, c->stream);
+    }
+    hipMemsetAsync(c->U[0], 0, field, c->stream);
+    hipMemsetAsync(c->U[1], 0, field, c->stream);
+    hipMemsetAsync(c->V, 0, field, c->stream);
+    hipMemsetAsync(c->A, 0, field, c->stream);
+    if ((ierr = dev_alloc((void **)&c->d_partial, sizeof(double) * 6 * 1024))) goto bad;
+  }
+  {  // empty source / receiver lists
+    std::vector<int> ptr(c->ntiles + 1, 0);
+    if ((ierr = upload<int>((void **)&c->d_tsrc_ptr, ptr.data(), ptr.size(), c->stream)) ||
+        (ierr = upload<int>((void **)&c->d_trec_ptr, ptr.data(), ptr.size(), c->stream)))
+      goto bad;
+  }
+  he = hipStreamSynchronize(c->stream);
+  if (he != hipSuccess) { ierr = fail(SE2W_ERR_HIP, "init: %s", hipGetErrorString(he)); goto bad; }
+  *out = c;
+  return SE2W_OK;
+bad:
+  se2w_destroy(c);
+  return ierr;
+}
+
+int se2w_destroy(se2w_ctx *c) {
+  if (!c) return SE2W_OK;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  void **bufs[] = {&c->U[0], &c->U[1], &c->C[0], &c->C[1], &c->V, &c->A, &c->F, &c->Utmp, &c->mass, &c->lambda,
+                   &c->mu, &c->edx, &c->edy, &c->D, &c->w2, &c->d_src_vals, &c->d_psi, &c->d_rec};
+  for (void **b : bufs) dev_free(*b);
+  int **ibufs[] = {&c->d_src_elem, &c->d_tsrc_ptr, &c->d_tsrc_idx, &c->d_rec_elem, &c->d_trec_ptr, &c->d_trec_idx};
+  for (int **b : ibufs) {
+    void *p = *b;
+    dev_free(p);
+    *b = nullptr;
+  }
+  if (c->d_partial) hipFree(c->d_partial);
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return SE2W_OK;
+}
+
+int se2w_set_tables(se2w_ctx *c, const double *D, const double *w1d, const double *elem_dx, const double *elem_dy) {
+  if (!c || !D || !w1d || !elem_dx || !elem_dy) return fail(SE2W_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const int nb = c->nb;
+  for (int e = 0; e < c->mx; e++)
+    if (!(elem_dx[e] > 0.0)) return fail(SE2W_ERR_ARG, "elem_dx[%d] = %g must be > 0", e, elem_dx[e]);
+  for (int e = 0; e < c->my; e++)
+    if (!(elem_dy[e] > 0.0)) return fail(SE2W_ERR_ARG, "elem_dy[%d] = %g must be > 0", e, elem_dy[e]);
+  std::vector<double> w2((size_t)nb * nb);
+  for (int j = 0; j < nb; j++)
+    for (int i = 0; i < nb; i++) w2[i + j * nb] = w1d[i] * w1d[j];  // se2wave.c:860-865
+  int ierr;
+  if ((ierr = upload_real(c, &c->D, D, (size_t)nb * nb)) || (ierr = upload_real(c, &c->w2, w2.data(), w2.size())) ||
+      (ierr = upload_real(c, &c->edx, elem_dx, c->mx)) || (ierr = upload_real(c, &c->edy, elem_dy, c->my)))
+    return ierr;
+  c->hD.assign(D, D + (size_t)nb * nb);
+  c->hdx.assign(elem_dx, elem_dx + c->mx);
+  c->hdy.assign(elem_dy, elem_dy + c->my);
+  c->have_tables = true;
+  return SE2W_OK;
+}
+
+int se2w_set_cells(se2w_ctx *c, const double *lambda, const double *mu) {
+  if (!c || !lambda || !mu) return fail(SE2W_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t ne = (size_t)c->mx * c->my;
+  for (size_t e = 0; e < ne; e++)
+    if (!std::isfinite(lambda[e]) || !std::isfinite(mu[e])) return fail(SE2W_ERR_ARG, "non-finite material in cell %zu", e);
+  int ierr;
+  if ((ierr = upload_real(c, &c->lambda, lambda, ne)) || (ierr = upload_real(c, &c->mu, mu, ne))) return ierr;
+  c->have_cells = true;
+  return SE2W_OK;
+}
+
+int se2w_set_mass(se2w_ctx *c, const double *mass) {
+  if (!c || !mass) return fail(SE2W_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const long n = (long)c->nx * c->ny;
+  for (long i = 0; i < n; i++)
+    if (!(mass[i] > 0.0)) return fail(SE2W_ERR_ARG, "mass[%ld] = %g must be > 0", i, mass[i]);
+  const size_t rs = real_size(c->mode);
+  int ierr = dev_alloc(&c->mass, (size_t)c->pitch * c->ny * rs);
+  if (ierr) return ierr;
+  double *tmp = nullptr;
+  if ((ierr = dev_alloc((void **)&tmp, sizeof(double) * n))) return ierr;
+  HIPCHK(hipMemcpyAsync(tmp, mass, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+  if (c->mode == SE2W_FP64)
+    hipLaunchKernelGGL(k_scalar_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp,
+                       (double *)c->mass, c->nx, c->ny, c->pitch, 0);
+  else  // 1/M is formed in FP64 and rounded once
+    hipLaunchKernelGGL(k_scalar_from_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (float *)c->mass,
+                       c->nx, c->ny, c->pitch, 1);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(tmp);
+  c->have_mass = true;
+  return SE2W_OK;
+}
+
+int se2w_set_sources(se2w_ctx *c, int32_t ns, const int32_t *elem, const double *vals) {
+  if (!c || ns < 0 || (ns > 0 && (!elem || !vals))) return fail(SE2W_ERR_ARG, "bad source arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const int ne = c->mx * c->my;
+  std::vector<int> el(elem, elem + ns);
+  for (int s = 0; s < ns; s++)
+    if (el[s] < 0 || el[s] >= ne) return fail(SE2W_ERR_ARG, "source %d: element %d out of range", s, el[s]);
+  std::vector<int> ptr, idx;
+  build_tile_lists(c, el, ptr, idx);
+  int ierr;
+  if ((ierr = upload<int>((void **)&c->d_src_elem, el.data(), el.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_tsrc_ptr, ptr.data(), ptr.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_tsrc_idx, idx.data(), idx.size(), c->stream)) ||
+      (ierr = upload_real(c, &c->d_src_vals, vals, (size_t)ns * 2 * c->npe)))
+    return ierr;
+  c->ns = ns;
+  return SE2W_OK;
+}
+
+int se2w_set_receivers(se2w_ctx *c, int32_t nr, const int32_t *elem, const int32_t *gll) {
+  if (!c || nr < 0 || (nr > 0 && (!elem || !gll))) return fail(SE2W_ERR_ARG, "bad receiver arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const int ne = c->mx * c->my;
+  for (int r = 0; r < nr; r++) {
+    if (elem[r] < 0 || elem[r] >= ne) return fail(SE2W_ERR_ARG, "receiver %d: element %d out of range", r, elem[r]);
+    if (gll[r] < 0 || gll[r] >= c->npe) return fail(SE2W_ERR_ARG, "receiver %d: gll %d out of range", r, gll[r]);
+  }
+  c->h_rec_elem.assign(elem, elem + nr);
+  c->h_rec_gll.assign(gll, gll + nr);
+  std::vector<int> ptr, idx;
+  build_tile_lists(c, c->h_rec_elem, ptr, idx);
+  int ierr;
+  if ((ierr = upload<int>((void **)&c->d_rec_elem, c->h_rec_elem.data(), nr, c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_trec_ptr, ptr.data(), ptr.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_trec_idx, idx.data(), idx.size(), c->stream)))
+    return ierr;
+  c->nr = nr;
+  c->rec_steps = 0;
+  c->trace.clear();
+  return SE2W_OK;
+}
+
+int se2w_set_state(se2w_ctx *c, const double *u, const double *v, const double *a) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  const long n = (long)c->nx * c->ny;
+  double *tmp = nullptr;
+  int ierr = dev_alloc((void **)&tmp, sizeof(double) * 2 * n);
+  if (ierr) return ierr;
+  const double *src[3] = {u, v, a};
+  void *dst[3] = {c->U[c->cur], c->V, c->A};
+  for (int f = 0; f < 3; f++) {
+    if (!src[f]) continue;
+    HIPCHK(hipMemcpyAsync(tmp, src[f], sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
+    if (c->mode == SE2W_FP64)
+      hipLaunchKernelGGL(k_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (double2 *)dst[f],
+                         c->nx, c->ny, c->pitch);
+    else
+      hipLaunchKernelGGL(k_from_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (float2 *)dst[f], c->nx,
+                         c->ny, c->pitch);
+    HIPCHK(hipGetLastError());
+  }
+  if (u && c->mode == SE2W_FP32_KAHAN)
+    HIPCHK(hipMemsetAsync(c->C[c->cur], 0, (size_t)c->pitch * c->ny * 2 * sizeof(float), c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(tmp);
+  return SE2W_OK;
+}
+
+int se2w_get_state(se2w_ctx *c, double *u, double *v, double *a) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  const long n = (long)c->nx * c->ny;
+  double *tmp = nullptr;
+  int ierr = dev_alloc((void **)&tmp, sizeof(double) * 2 * n);
+  if (ierr) return ierr;
+  double *dst[3] = {u, v, a};
+  void *src[3] = {c->U[c->cur], c->V, c->A};
+  for (int f = 0; f < 3; f++) {
+    if (!dst[f]) continue;
+    if (c->mode == SE2W_FP64)
+      hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)src[f],
+                         (const double2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
+    else
+      hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)src[f],
+                         (const float2 *)((f == 0 && c->mode == SE2W_FP32_KAHAN) ? c->C[c->cur] : nullptr), tmp,
+                         c->nx, c->ny, c->pitch);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(dst[f], tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(tmp);
+  return SE2W_OK;
+}
+
+int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  if (nsteps < 1) return fail(SE2W_ERR_ARG, "nsteps must be >= 1");
+  if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SE2W_ERR_ARG, "dt must be positive and finite");
+  if (c->ns > 0 && !psi) return fail(SE2W_ERR_ARG, "psi table required when sources are set");
+  if (!c->have_tables || !c->have_cells || !c->have_mass)
+    return fail(SE2W_ERR_STATE, "set tables, cells and mass before se2w_run");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t rs = real_size(c->mode);
+  int ierr;
+  // source-time table for this run
+  if (c->ns > 0) {
+    const size_t n = (size_t)nsteps * c->ns;
+    if (n > c->psi_cap) {
+      dev_free(c->d_psi);
+      if ((ierr = dev_alloc(&c->d_psi, n * rs))) return ierr;
+      c->psi_cap = n;
+    }
+    if (c->mode == SE2W_FP64) {
+      HIPCHK(hipMemcpyAsync(c->d_psi, psi, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    } else {
+      std::vector<float> tmp(n);
+      for (size_t i = 0; i < n; i++) tmp[i] = (float)psi[i];
+      HIPCHK(hipMemcpyAsync(c->d_psi, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipStreamSynchronize(c->stream));
+    }
+  }
+  // receiver sample buffer
+  if (c->nr > 0) {
+    const size_t n = (size_t)nsteps * c->nr * c->npe * 6;
+    if (n > c->rec_cap) {
+      dev_free(c->d_rec);
+      if ((ierr = dev_alloc(&c->d_rec, n * rs))) return ierr;
+      c->rec_cap = n;
+    }
+  }
+  c->rec_steps = nsteps;
+  c->trace.clear();
+  if (c->mode == SE2W_FP64) ierr = run_mode<MODE_F64>(c, nsteps, dt);
+  else if (c->mode == SE2W_FP32) ierr = run_mode<MODE_F32>(c, nsteps, dt);
+  else ierr = run_mode<MODE_F32K>(c, nsteps, dt);
+  return ierr;
+}
+
+int se2w_get_receivers(se2w_ctx *c, double *trace) {
+#pragma clang fp contract(off)
+  if (!c || (!trace && c->nr > 0)) return fail(SE2W_ERR_ARG, "null argument");
+  if (c->nr == 0 || c->rec_steps == 0) return SE2W_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const int nb = c->nb, npe = c->npe, nr = c->nr;
+  const size_t n = (size_t)c->rec_steps * nr * npe * 6;
+  std::vector<double> rec(n);
+  if (c->mode == SE2W_FP64) {
+    HIPCHK(hipMemcpyAsync(rec.data(), c->d_rec, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), c->d_rec, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) rec[i] = (double)tmp[i];
+  }
+  for (int k = 0; k < c->rec_steps; k++) {
+    for (int r = 0; r < nr; r++) {
+      const double *R = &rec[((size_t)k * nr + r) * npe * 6];
+      double *o = &trace[((size_t)k * nr + r) * 7];
+      const int g = c->h_rec_gll[r], gi = g % nb, gj = g / nb;
+      for (int q = 0; q < 6; q++) o[q] = R[g * 6 + q];
+      // curl(v) at the receiver's GLL node (se2wave.c:2922-2959): dN_dx on
+      // row gj, dN_dy on column gi, k ascending.
+      const int ex = c->h_rec_elem[r] % c->mx, ey = c->h_rec_elem[r] / c->mx;
+      const double J00 = 0.5 * c->hdx[ex], J11 = 0.5 * c->hdy[ey];
+      const double iJ00 = 1.0 / J00, iJ11 = 1.0 / J11;
+      double dvxdy = 0.0, dvydx = 0.0;
+      for (int j = 0; j < nb; j++) {
+        for (int i = 0; i < nb; i++) {
+          const double *node = &R[(i + j * nb) * 6];
+          if (i == gi) dvxdy = dvxdy + (iJ11 * c->hD[gj * nb + j]) * node[2];
+          if (j == gj) dvydx = dvydx + (iJ00 * c->hD[gi * nb + i]) * node[3];
+        }
+      }
+      o[6] = dvydx - dvxdy;
+    }
+  }
+  return SE2W_OK;
+}
+
+int se2w_assemble_forces(se2w_ctx *c, const double *u, double *f) {
+  if (!c || !u || !f) return fail(SE2W_ERR_ARG, "null argument");
+  if (!c->have_tables || !c->have_cells) return fail(SE2W_ERR_STATE, "set tables and cells first");
+  HIPCHK(hipSetDevice(c->device));
+  const size_t rs = real_size(c->mode);
+  const size_t field = (size_t)c->pitch * c->ny * 2 * rs;
+  int ierr;
+  if ((ierr = dev_alloc(&c->Utmp, field))) return ierr;
+  if ((ierr = dev_alloc(&c->F, field))) return ierr;
+  const long n = (long)c->nx * c->ny;
+  double *tmp = nullptr;
+  if ((ierr = dev_alloc((void **)&tmp, sizeof(double) * 2 * n))) return ierr;
+  HIPCHK(hipMemcpyAsync(tmp, u, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
+  if (c->mode == SE2W_FP64) {
+    hipLaunchKernelGGL(k_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (double2 *)c->Utmp,
+                       c->nx, c->ny, c->pitch);
+    StepParams<double> prm = make_params<MODE_F64>(c, 0, 1.0, 0);
+    prm.u_in = (const double2 *)c->Utmp;
+    launch_element_only<MODE_F64>(c, prm);
+    hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)c->F,
+                       (const double2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
+  } else {
+    hipLaunchKernelGGL(k_from_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (float2 *)c->Utmp, c->nx,
+                       c->ny, c->pitch);
+    StepParams<float> prm = make_params<MODE_F32>(c, 0, 1.0, 0);
+    prm.u_in = (const float2 *)c->Utmp;
+    launch_element_only<MODE_F32>(c, prm);
+    hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)c->F,
+                       (const float2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(f, tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  hipFree(tmp);
+  return SE2W_OK;
+}
+
+int se2w_norms(se2w_ctx *c, double out[6]) {
+  if (!c || !out) return fail(SE2W_ERR_ARG, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const int blocks = 1024;
+  if (c->mode == SE2W_FP64)
+    hipLaunchKernelGGL(k_norms<double>, dim3(blocks), dim3(256), 0, c->stream, (const double2 *)c->U[c->cur],
+                       (const double2 *)c->V, c->nx, c->ny, c->pitch, c->d_partial);
+  else
+    hipLaunchKernelGGL(k_norms<float>, dim3(blocks), dim3(256), 0, c->stream, (const float2 *)c->U[c->cur],
+                       (const float2 *)c->V, c->nx, c->ny, c->pitch, c->d_partial);
+  HIPCHK(hipGetLastError());
+  std::vector<double> part((size_t)blocks * 6);
+  HIPCHK(hipMemcpyAsync(part.data(), c->d_partial, part.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double r[6] = {-1.0e300, 1.0e300, 0.0, -1.0e300, 1.0e300, 0.0};
+  for (int b = 0; b < blocks; b++) {
+    r[0] = std::max(r[0], part[b * 6 + 0]);
+    r[1] = std::min(r[1], part[b * 6 + 1]);
+    r[2] += part[b * 6 + 2];
+    r[3] = std::max(r[3], part[b * 6 + 3]);
+    r[4] = std::min(r[4], part[b * 6 + 4]);
+    r[5] += part[b * 6 + 5];
+  }
+  out[0] = r[0]; out[1] = r[1]; out[2] = std::sqrt(r[2]);
+  out[3] = r[3]; out[4] = r[4]; out[5] = std::sqrt(r[5]);
+  return SE2W_OK;
+}
+
+int se2w_set_profiling(se2w_ctx *c, int32_t on) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  c->profiling = on != 0;
+  return SE2W_OK;
+}
+
+int se2w_get_timing(se2w_ctx *c, double *run_ms, double kernel_ms[2]) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  if (run_ms) *run_ms = c->last_ms;
+  if (kernel_ms) { kernel_ms[0] = c->kern_ms[0]; kernel_ms[1] = c->kern_ms[1]; }
+  return SE2W_OK;
+}
+
+int se2w_get_tile(se2w_ctx *c, int32_t *tex, int32_t *tey) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  if (tex) *tex = c->tex;
+  if (tey) *tey = c->tey;
+  return SE2W_OK;
+}
+
+}  // extern "C"

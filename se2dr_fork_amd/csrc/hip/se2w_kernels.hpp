@@ -1,0 +1,505 @@
+// se2w_kernels.hpp -- CDNA4 (gfx950) kernels for se2wave's explicit
+// spectral-element time step.
+//
+// Storage (HBM): nodal fields are interleaved 2-vectors (x, y) in DMDA node
+// order (node = I + J*nx, se2wave.c:706-736) with the row pitch padded to a
+// multiple of 32 nodes so that every tile row starts on a 128-byte line.
+// Per-cell lambda/mu are indexed e = ei + ej*mx.  Element widths depend only
+// on the element column (elem_dx[ei]) or row (elem_dy[ej]) because the
+// reference's mesh is a tensor product (se2wave.c:739-779).
+//
+// Work decomposition: one workgroup per tile of TEX x TEY elements.  The
+// workgroup owns the nodes I in [ex0*P, ex_end*P) (+ the last node column at
+// the right boundary) and likewise in y.  To finish the internal force of its
+// left/bottom boundary nodes it also evaluates the one element column to the
+// left and the element row below (halo elements).  Every owned node then sums
+// the contributions of its (up to four) elements in ascending element index,
+// the accumulation order of the reference's assembly (se2wave.c:1417-1422),
+// so the result is deterministic and, in FP64 mode, bit-identical to it.
+//
+// Modes: MODE_F64 reproduces the reference's FP64 operation order with FMA
+// contraction disabled; MODE_F32 evaluates the sum-factorised operator in FP32
+// with FMA; MODE_F32K adds a compensated (Kahan) displacement accumulation.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace se2w {
+
+enum { MODE_F64 = 0, MODE_F32 = 1, MODE_F32K = 2 };
+
+template <int MODE> struct RealOf { using T = float; };
+template <> struct RealOf<MODE_F64> { using T = double; };
+
+template <typename R> struct V2 { using T = float2; };
+template <> struct V2<double> { using T = double2; };
+
+// Tile shapes: the largest tile whose LDS footprint (u tile + stress tile)
+// stays under 48 KB, favouring few halo elements.
+template <int P, typename R> struct TileShape;
+#define SE2W_TILE(P_, R_, X_, Y_) \
+  template <> struct TileShape<P_, R_> { static constexpr int TEX = X_, TEY = Y_; };
+SE2W_TILE(1, float, 32, 24)
+SE2W_TILE(2, float, 24, 12)
+SE2W_TILE(3, float, 12, 12)
+SE2W_TILE(4, float, 10, 8)
+SE2W_TILE(5, float, 8, 6)
+SE2W_TILE(6, float, 8, 5)
+SE2W_TILE(7, float, 5, 5)
+SE2W_TILE(8, float, 5, 4)
+SE2W_TILE(1, double, 24, 16)
+SE2W_TILE(2, double, 12, 12)
+SE2W_TILE(3, double, 12, 6)
+SE2W_TILE(4, double, 8, 5)
+SE2W_TILE(5, double, 5, 5)
+SE2W_TILE(6, double, 4, 4)
+SE2W_TILE(7, double, 4, 3)
+SE2W_TILE(8, double, 3, 3)
+#undef SE2W_TILE
+
+constexpr int kThreads = 256;
+constexpr int kMaxP = 8;
+
+template <typename R> struct StepParams {
+  using R2 = typename V2<R>::T;
+  int mx, my, nx, ny, pitch;
+  int tiles_x, tiles_y, tiles_per_xcd;
+  const R2 *u_in;  // displacement read by the element operator
+  R2 *u_out;       // displacement written by the update (may alias u_in in split mode)
+  R2 *v;           // velocity, updated in place
+  R2 *a;           // acceleration, written when `last`
+  R2 *f;           // split mode: internal force between the two kernels
+  const R2 *c_in;  // Kahan compensation (MODE_F32K)
+  R2 *c_out;
+  const R *mass;   // MODE_F64: lumped mass M; FP32 modes: 1/M (per node, padded)
+  const R *lambda, *mu;
+  const R *elem_dx, *elem_dy;
+  const R *D;      // D[q*(P+1) + i] = dN_i/dxi at GLL node q
+  const R *w2;     // w2[i + j*(P+1)] = w_i * w_j
+  // sources (CSR per tile)
+  const int *tile_src_ptr, *tile_src_idx, *src_elem;
+  const R *src_vals;  // [ns][2*npe]
+  const R *psi;       // [nsteps_run][ns]
+  int ns;
+  // receivers (CSR per tile)
+  const int *tile_rec_ptr, *tile_rec_idx, *rec_elem;
+  R *rec;             // [nsteps_run][nr][npe][6]
+  int nr;
+  int step;           // step index inside the current run
+  R dt, hdt, hdt2;    // dt, 0.5*dt, 0.5*dt*dt (formed in FP64 on the host)
+  int last;           // 1: keep the corrected state (no predictor), write a
+};
+
+// XCD-aware tile order: hardware places workgroup b on XCD (b mod 8).  Give
+// each XCD a contiguous band of tiles so that halo re-reads of a neighbouring
+// tile hit the same XCD's L2.
+__device__ __forceinline__ int tile_of_block(int b, int tiles_per_xcd) {
+  return (b & 7) * tiles_per_xcd + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Phase B: stress at every GLL point of every tile element.
+// ---------------------------------------------------------------------------
+
+// FP64, operation order of the reference's dense loop (se2wave.c:1357-1400),
+// with the zero products of the padded tensor tables left out (they only add
+// signed zeros).  dN_dx = iJ00*D[qi][i] on row qj, dN_dy = iJ11*D[qj][j] on
+// column qi.
+template <int P>
+__device__ __forceinline__ void stress_f64(const double *ux_row, const double *ux_col, const double *uy_row,
+                                           const double *uy_col, int pitch, int qi, int qj, const double *D,
+                                           double iJ00, double iJ11, double lam, double mu, double &sxx,
+                                           double &syy, double &sxy) {
+#pragma clang fp contract(off)
+  constexpr int NB = P + 1;
+  double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
+  for (int i = 0; i < NB; i++) e0 = e0 + (iJ00 * D[qi * NB + i]) * ux_row[i];
+#pragma unroll
+  for (int j = 0; j < NB; j++) e1 = e1 + (iJ11 * D[qj * NB + j]) * uy_col[j * pitch];
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    if (j != qj) {
+      e2 = e2 + (iJ11 * D[qj * NB + j]) * ux_col[j * pitch];
+    } else {
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        double t = (iJ00 * D[qi * NB + i]) * uy_row[i];
+        if (i == qi) t = t + (iJ11 * D[qj * NB + qj]) * ux_col[qj * pitch];
+        e2 = e2 + t;
+      }
+    }
+  }
+  const double c11 = 2.0 * mu + lam, c12 = lam, c21 = lam, c22 = 2.0 * mu + lam, c33 = mu;
+  sxx = c11 * e0 + c12 * e1;
+  syy = c21 * e0 + c22 * e1;
+  sxy = c33 * e2;
+}
+
+// FP32: sum-factorised contractions with the Jacobian applied once, stress
+// pre-multiplied by the quadrature factor detJ*w_q.
+template <int P>
+__device__ __forceinline__ void stress_f32(const float *ux_row, const float *ux_col, const float *uy_row,
+                                           const float *uy_col, int pitch, int qi, int qj, const float *D,
+                                           float iJ00, float iJ11, float lam, float mu, float fac, float &sxx,
+                                           float &syy, float &sxy) {
+  constexpr int NB = P + 1;
+  float dxux = 0.f, dxuy = 0.f, dyux = 0.f, dyuy = 0.f;
+#pragma unroll
+  for (int i = 0; i < NB; i++) {
+    const float d = D[qi * NB + i];
+    dxux = fmaf(d, ux_row[i], dxux);
+    dxuy = fmaf(d, uy_row[i], dxuy);
+  }
+#pragma unroll
+  for (int j = 0; j < NB; j++) {
+    const float d = D[qj * NB + j];
+    dyux = fmaf(d, ux_col[j * pitch], dyux);
+    dyuy = fmaf(d, uy_col[j * pitch], dyuy);
+  }
+  const float e0 = iJ00 * dxux, e1 = iJ11 * dyuy, e2 = iJ11 * dyux + iJ00 * dxuy;
+  const float c11 = 2.f * mu + lam;
+  sxx = fac * (c11 * e0 + lam * e1);
+  syy = fac * (lam * e0 + c11 * e1);
+  sxy = fac * (mu * e2);
+}
+
+// ---------------------------------------------------------------------------
+// Phase C: internal force of node (i,j) of one element from the stresses.
+// ---------------------------------------------------------------------------
+
+// FP64: fe[2k] += -fac*(dNdx*sxx + dNdy*sxy); fe[2k+1] += -fac*(dNdy*syy +
+// dNdx*sxy) over q ascending (se2wave.c:1402-1413); nonzero terms only.
+template <int P>
+__device__ __forceinline__ void force_f64(const double *Sxx, const double *Syy, const double *Sxy, int qpitch, int i,
+                                          int j, const double *D, const double *w2, double iJ00, double iJ11,
+                                          double detJ, double &fx, double &fy) {
+#pragma clang fp contract(off)
+  constexpr int NB = P + 1;
+  double ax = 0.0, ay = 0.0;
+#pragma unroll
+  for (int qj = 0; qj < NB; qj++) {
+    if (qj != j) {
+      const int q = i + qj * qpitch;
+      const double fac = detJ * w2[i + qj * NB];
+      const double dNy = iJ11 * D[qj * NB + j];
+      ax = ax + (-fac) * (dNy * Sxy[q]);
+      ay = ay + (-fac) * (dNy * Syy[q]);
+    } else {
+#pragma unroll
+      for (int qi = 0; qi < NB; qi++) {
+        const int q = qi + j * qpitch;
+        const double fac = detJ * w2[qi + j * NB];
+        const double dNx = iJ00 * D[qi * NB + i];
+        if (qi != i) {
+          ax = ax + (-fac) * (dNx * Sxx[q]);
+          ay = ay + (-fac) * (dNx * Sxy[q]);
+        } else {
+          const double dNy = iJ11 * D[j * NB + j];
+          ax = ax + (-fac) * (dNx * Sxx[q] + dNy * Sxy[q]);
+          ay = ay + (-fac) * (dNy * Syy[q] + dNx * Sxy[q]);
+        }
+      }
+    }
+  }
+  fx = ax;
+  fy = ay;
+}
+
+// FP32: transposed contractions of the fac-weighted stresses.
+template <int P>
+__device__ __forceinline__ void force_f32(const float *Sxx, const float *Syy, const float *Sxy, int qpitch, int i,
+                                          int j, const float *D, float iJ00, float iJ11, float &fx, float &fy) {
+  constexpr int NB = P + 1;
+  float rx = 0.f, ry = 0.f, cx = 0.f, cy = 0.f;
+#pragma unroll
+  for (int qi = 0; qi < NB; qi++) {
+    const float d = D[qi * NB + i];
+    rx = fmaf(d, Sxx[qi + j * qpitch], rx);
+    ry = fmaf(d, Sxy[qi + j * qpitch], ry);
+  }
+#pragma unroll
+  for (int qj = 0; qj < NB; qj++) {
+    const float d = D[qj * NB + j];
+    cx = fmaf(d, Sxy[i + qj * qpitch], cx);
+    cy = fmaf(d, Syy[i + qj * qpitch], cy);
+  }
+  fx = -(iJ00 * rx + iJ11 * cx);
+  fy = -(iJ00 * ry + iJ11 * cy);
+}
+
+// ---------------------------------------------------------------------------
+// Phase D: corrector of step n+1, receiver sampling, predictor of step n+2.
+// Sequence per node (se2wave.c:4739-4745 then 4726-4730):
+//   f = f + g;  a = f / M;  v = v + (0.5 dt) a;  [record u, v, a]
+//   u = u + dt v;  u = u + (0.5 dt^2) a;  v = v + (0.5 dt) a.
+// ---------------------------------------------------------------------------
+template <int P, int MODE>
+__device__ __forceinline__ void node_update(const StepParams<typename RealOf<MODE>::T> &prm, int tile, int gx,
+                                            int gy, typename RealOf<MODE>::T fx, typename RealOf<MODE>::T fy,
+                                            typename RealOf<MODE>::T ux, typename RealOf<MODE>::T uy) {
+#pragma clang fp contract(off)
+  using R = typename RealOf<MODE>::T;
+  using R2 = typename V2<R>::T;
+  constexpr int NB = P + 1;
+  const long idx = (long)gy * prm.pitch + gx;
+  // g = sum_s psi_s * vals_s in source order, then f = f + g (3476-3495, 4739)
+  {
+    const int s0 = prm.tile_src_ptr[tile], s1 = prm.tile_src_ptr[tile + 1];
+    if (s0 < s1) {
+      R gxv = R(0), gyv = R(0);
+      bool hit = false;
+      for (int t = s0; t < s1; t++) {
+        const int s = prm.tile_src_idx[t];
+        const int e = prm.src_elem[s];
+        const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
+        if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
+          const int k = li + lj * NB;
+          const R ps = prm.psi[(long)prm.step * prm.ns + s];
+          gxv = gxv + ps * prm.src_vals[(long)s * 2 * NB * NB + 2 * k];
+          gyv = gyv + ps * prm.src_vals[(long)s * 2 * NB * NB + 2 * k + 1];
+          hit = true;
+        }
+      }
+      if (hit) {
+        fx = fx + gxv;
+        fy = fy + gyv;
+      }
+    }
+  }
+  R ax, ay;
+  if constexpr (MODE == MODE_F64) {
+    const R M = prm.mass[idx];
+    ax = fx / M;
+    ay = fy / M;
+  } else {
+    const R mi = prm.mass[idx];
+    ax = fx * mi;
+    ay = fy * mi;
+  }
+  R2 vv = prm.v[idx];
+  R vx = vv.x + prm.hdt * ax;
+  R vy = vv.y + prm.hdt * ay;
+  // receivers: every node of a receiver's element records (u, v, a)
+  {
+    const int r0 = prm.tile_rec_ptr[tile], r1 = prm.tile_rec_ptr[tile + 1];
+    for (int t = r0; t < r1; t++) {
+      const int r = prm.tile_rec_idx[t];
+      const int e = prm.rec_elem[r];
+      const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
+      if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
+        R *o = prm.rec + (((long)prm.step * prm.nr + r) * (NB * NB) + li + lj * NB) * 6;
+        o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy; o[4] = ax; o[5] = ay;
+      }
+    }
+  }
+  if (prm.last) {
+    prm.u_out[idx] = R2{ux, uy};
+    prm.v[idx] = R2{vx, vy};
+    prm.a[idx] = R2{ax, ay};
+    if constexpr (MODE == MODE_F32K) prm.c_out[idx] = prm.c_in[idx];
+    return;
+  }
+  R nux, nuy;
+  if constexpr (MODE == MODE_F32K) {
+    // two compensated additions mirroring the reference's two AXPYs
+    R2 c = prm.c_in[idx];
+    R y, t;
+    y = prm.dt * vx - c.x; t = ux + y; c.x = (t - ux) - y; nux = t;
+    y = prm.dt * vy - c.y; t = uy + y; c.y = (t - uy) - y; nuy = t;
+    R mx_ = nux, my_ = nuy;
+    y = prm.hdt2 * ax - c.x; t = mx_ + y; c.x = (t - mx_) - y; nux = t;
+    y = prm.hdt2 * ay - c.y; t = my_ + y; c.y = (t - my_) - y; nuy = t;
+    prm.c_out[idx] = c;
+  } else {
+    nux = ux + prm.dt * vx;
+    nuy = uy + prm.dt * vy;
+    nux = nux + prm.hdt2 * ax;
+    nuy = nuy + prm.hdt2 * ay;
+  }
+  prm.u_out[idx] = R2{nux, nuy};
+  prm.v[idx] = R2{vx + prm.hdt * ax, vy + prm.hdt * ay};
+}
+
+// ---------------------------------------------------------------------------
+// Tile kernel.  FUSED: element operator + update in one pass (one launch per
+// step).  !FUSED: element operator only, f written for the update kernel.
+// ---------------------------------------------------------------------------
+template <int P, int MODE, bool FUSED>
+__global__ __launch_bounds__(kThreads) void tile_kernel(StepParams<typename RealOf<MODE>::T> prm) {
+  using R = typename RealOf<MODE>::T;
+  using R2 = typename V2<R>::T;
+  constexpr int NB = P + 1;
+  constexpr int TEX = TileShape<P, R>::TEX, TEY = TileShape<P, R>::TEY;
+  constexpr int NXL = (TEX + 1) * P + 1, NYL = (TEY + 1) * P + 1;
+  constexpr int QX = (TEX + 1) * NB, QY = (TEY + 1) * NB;
+  __shared__ R su[2][NYL][NXL];
+  __shared__ R ss[3][QY][QX];
+  __shared__ R sD[NB * NB], sW[NB * NB];
+
+  const int ntiles = prm.tiles_x * prm.tiles_y;
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= ntiles) return;  // uniform per workgroup
+  const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
+  const int ex0 = tx * TEX, ey0 = ty * TEY;
+  const int gx0 = (ex0 - 1) * P, gy0 = (ey0 - 1) * P;
+  const int tid = threadIdx.x;
+
+  for (int k = tid; k < NB * NB; k += kThreads) {
+    sD[k] = prm.D[k];
+    sW[k] = prm.w2[k];
+  }
+  // Phase A: displacement of the tile's nodes (+ halo) into LDS.
+  for (int k = tid; k < NXL * NYL; k += kThreads) {
+    const int ly = k / NXL, lx = k - ly * NXL;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    R2 uv = R2{R(0), R(0)};
+    if (gx >= 0 && gx < prm.nx && gy >= 0 && gy < prm.ny) uv = prm.u_in[(long)gy * prm.pitch + gx];
+    su[0][ly][lx] = uv.x;
+    su[1][ly][lx] = uv.y;
+  }
+  __syncthreads();
+  // Phase B: stress at the GLL points of the tile elements (+ halo).
+  for (int k = tid; k < QX * QY; k += kThreads) {
+    const int qy = k / QX, qx = k - qy * QX;
+    const int elx = qx / NB, qi = qx - elx * NB;
+    const int ely = qy / NB, qj = qy - ely * NB;
+    const int ex = ex0 - 1 + elx, ey = ey0 - 1 + ely;
+    if (ex < 0 || ex >= prm.mx || ey < 0 || ey >= prm.my) continue;
+    const int e = ex + ey * prm.mx;
+    const int lx0 = elx * P, ly0 = ely * P;
+    const R *uxr = &su[0][ly0 + qj][lx0];
+    const R *uyr = &su[1][ly0 + qj][lx0];
+    const R *uxc = &su[0][ly0][lx0 + qi];
+    const R *uyc = &su[1][ly0][lx0 + qi];
+    R sxx, syy, sxy;
+    if constexpr (MODE == MODE_F64) {
+#pragma clang fp contract(off)
+      const double J00 = 0.5 * prm.elem_dx[ex], J11 = 0.5 * prm.elem_dy[ey];
+      const double iJ00 = 1.0 / J00, iJ11 = 1.0 / J11;
+      stress_f64<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, iJ00, iJ11, prm.lambda[e], prm.mu[e], sxx, syy, sxy);
+    } else {
+      const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
+      const float fac = (J00 * J11) * sW[qi + qj * NB];
+      stress_f32<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, 1.f / J00, 1.f / J11, prm.lambda[e], prm.mu[e], fac, sxx,
+                    syy, sxy);
+    }
+    ss[0][qy][qx] = sxx;
+    ss[1][qy][qx] = syy;
+    ss[2][qy][qx] = sxy;
+  }
+  __syncthreads();
+  // Phase C (+D): owned nodes.
+  const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
+  const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
+  const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
+  for (int k = tid; k < OX * OY; k += kThreads) {
+    const int oy = k / OX, ox = k - oy * OX;
+    const int gx = ex0 * P + ox, gy = ey0 * P + oy;
+    int exs[2], eys[2], nex = 0, ney = 0;
+    if (gx % P == 0) {
+      if (gx / P - 1 >= 0) exs[nex++] = gx / P - 1;
+      if (gx / P < prm.mx) exs[nex++] = gx / P;
+    } else {
+      exs[nex++] = gx / P;
+    }
+    if (gy % P == 0) {
+      if (gy / P - 1 >= 0) eys[ney++] = gy / P - 1;
+      if (gy / P < prm.my) eys[ney++] = gy / P;
+    } else {
+      eys[ney++] = gy / P;
+    }
+    R fx = R(0), fy = R(0);
+    for (int b = 0; b < ney; b++) {
+      for (int a = 0; a < nex; a++) {
+        const int ex = exs[a], ey = eys[b];
+        const int elx = ex - ex0 + 1, ely = ey - ey0 + 1;
+        const int i = gx - ex * P, j = gy - ey * P;
+        const R *Sxx = &ss[0][ely * NB][elx * NB];
+        const R *Syy = &ss[1][ely * NB][elx * NB];
+        const R *Sxy = &ss[2][ely * NB][elx * NB];
+        R cfx, cfy;
+        if constexpr (MODE == MODE_F64) {
+#pragma clang fp contract(off)
+          const double J00 = 0.5 * prm.elem_dx[ex], J11 = 0.5 * prm.elem_dy[ey];
+          const double iJ00 = 1.0 / J00, iJ11 = 1.0 / J11, detJ = J00 * J11;
+          force_f64<P>(Sxx, Syy, Sxy, QX, i, j, sD, sW, iJ00, iJ11, detJ, cfx, cfy);
+          fx = fx + cfx;
+          fy = fy + cfy;
+        } else {
+          const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
+          force_f32<P>(Sxx, Syy, Sxy, QX, i, j, sD, 1.f / J00, 1.f / J11, cfx, cfy);
+          fx += cfx;
+          fy += cfy;
+        }
+      }
+    }
+    if constexpr (FUSED) {
+      const int lx = gx - gx0, ly = gy - gy0;
+      node_update<P, MODE>(prm, tile, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
+    } else {
+      prm.f[(long)gy * prm.pitch + gx] = R2{fx, fy};
+    }
+  }
+}
+
+// Update kernel of the split schedule: one workgroup per tile of owned nodes.
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void update_kernel(StepParams<typename RealOf<MODE>::T> prm) {
+  using R = typename RealOf<MODE>::T;
+  using R2 = typename V2<R>::T;
+  constexpr int TEX = TileShape<P, R>::TEX, TEY = TileShape<P, R>::TEY;
+  const int ntiles = prm.tiles_x * prm.tiles_y;
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= ntiles) return;
+  const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
+  const int ex0 = tx * TEX, ey0 = ty * TEY;
+  const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
+  const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
+  const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
+  for (int k = threadIdx.x; k < OX * OY; k += kThreads) {
+    const int oy = k / OX, ox = k - oy * OX;
+    const int gx = ex0 * P + ox, gy = ey0 * P + oy;
+    const long idx = (long)gy * prm.pitch + gx;
+    const R2 f = prm.f[idx];
+    const R2 u = prm.u_in[idx];
+    node_update<P, MODE>(prm, tile, gx, gy, f.x, f.y, u.x, u.y);
+  }
+}
+
+// Predictor that opens a run from a plain state (u_n, v_n, a_n):
+// u* = u + dt v + (0.5 dt^2) a;  v* = v + (0.5 dt) a   (se2wave.c:4726-4730).
+template <int MODE>
+__global__ __launch_bounds__(kThreads) void predictor_kernel(StepParams<typename RealOf<MODE>::T> prm) {
+#pragma clang fp contract(off)
+  using R = typename RealOf<MODE>::T;
+  using R2 = typename V2<R>::T;
+  const long n = (long)prm.pitch * prm.ny;
+  for (long idx = (long)blockIdx.x * kThreads + threadIdx.x; idx < n; idx += (long)gridDim.x * kThreads) {
+    const int gx = (int)(idx % prm.pitch);
+    if (gx >= prm.nx) continue;
+    const R2 u = prm.u_in[idx];
+    const R2 v = prm.v[idx];
+    const R2 a = prm.a[idx];
+    R nux, nuy;
+    if constexpr (MODE == MODE_F32K) {
+      R2 c = prm.c_in[idx];
+      R y, t;
+      y = prm.dt * v.x - c.x; t = u.x + y; c.x = (t - u.x) - y; nux = t;
+      y = prm.dt * v.y - c.y; t = u.y + y; c.y = (t - u.y) - y; nuy = t;
+      R m0 = nux, m1 = nuy;
+      y = prm.hdt2 * a.x - c.x; t = m0 + y; c.x = (t - m0) - y; nux = t;
+      y = prm.hdt2 * a.y - c.y; t = m1 + y; c.y = (t - m1) - y; nuy = t;
+      prm.c_out[idx] = c;
+    } else {
+      nux = u.x + prm.dt * v.x;
+      nuy = u.y + prm.dt * v.y;
+      nux = nux + prm.hdt2 * a.x;
+      nuy = nuy + prm.hdt2 * a.y;
+    }
+    prm.u_out[idx] = R2{nux, nuy};
+    prm.v[idx] = R2{v.x + prm.hdt * a.x, v.y + prm.hdt * a.y};
+  }
+}
+
+}  // namespace se2w

@@ -1,0 +1,145 @@
+/*
+ * solver.c -- drives the GPU step (include/se2w.h) from the host objects:
+ * the se2wave_demo time loop (se2wave.c:4722-4775) with the source-time
+ * functions evaluated on the host at the accumulated time
+ * (time = time + dt, 4724) and the receiver rows collected per advance.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "host_internal.h"
+
+struct se2w_solver {
+  struct se2w_spec *spec;
+  se2w_ctx *ctx;
+  int ns;            /* sources handed to the GPU (those with a source and an STF) */
+  SE2WSource *src;   /* [ns] */
+  SE2WSTF *stf;      /* [ns] */
+  SE2WReceivers recv;
+  double time;
+  int steps;
+  int last_n;
+  double *times, *trace, *psi;
+  size_t times_cap, trace_cap, psi_cap;
+};
+
+static int gpu_fail(int code)
+{
+  return host_fail(code, "GPU: %s (%s)", se2w_strerror(code), se2w_last_error());
+}
+
+int se2w_solver_create(SE2WSpec spec, int precision, int schedule, int ns, SE2WSource *src, SE2WSTF *stf,
+                       SE2WReceivers recv, SE2WSolver *out)
+{
+  struct se2w_solver *sv;
+  se2w_desc desc;
+  double D[HOST_MAX_NB * HOST_MAX_NB], w1d[HOST_MAX_NB], *edx = NULL, *edy = NULL, *mass = NULL, *vals = NULL;
+  int32_t *elem = NULL;
+  int ierr, p, k;
+  if (!spec || !out || ns < 0) return host_fail(SE2W_HOST_ERR_ARG, "bad solver arguments");
+  *out = NULL;
+  if (!spec->have_material) return host_fail(SE2W_HOST_ERR_USER, "material not set");
+  sv = (struct se2w_solver *)calloc(1, sizeof(*sv));
+  if (!sv) return host_fail(SE2W_HOST_ERR_MEM, "out of memory");
+  sv->spec = spec;
+  sv->recv = recv;
+  sv->src = (SE2WSource *)calloc(ns > 0 ? ns : 1, sizeof(SE2WSource));
+  sv->stf = (SE2WSTF *)calloc(ns > 0 ? ns : 1, sizeof(SE2WSTF));
+  if (!sv->src || !sv->stf) { ierr = host_fail(SE2W_HOST_ERR_MEM, "out of memory"); goto bad; }
+  /* SeismicSourceEvaluate skips a source without an STF (3485-3490) */
+  for (p = 0; p < ns; p++) {
+    if (!src || !src[p] || !stf || !stf[p]) continue;
+    if (!src[p]->issetup) { ierr = host_fail(SE2W_HOST_ERR_USER, "Seismic source not setup. Must call SeismicSourceSetup()"); goto bad; }
+    sv->src[sv->ns] = src[p];
+    sv->stf[sv->ns] = stf[p];
+    sv->ns++;
+  }
+  desc.mx = spec->mx; desc.my = spec->my; desc.p = spec->p;
+  desc.precision = precision; desc.schedule = schedule; desc.device = 0;
+  if ((ierr = se2w_create(&desc, &sv->ctx))) { ierr = gpu_fail(ierr); goto bad; }
+  edx = (double *)malloc(sizeof(double) * spec->mx);
+  edy = (double *)malloc(sizeof(double) * spec->my);
+  mass = (double *)malloc(sizeof(double) * (size_t)spec->nnodes);
+  if (!edx || !edy || !mass) { ierr = host_fail(SE2W_HOST_ERR_MEM, "out of memory"); goto bad; }
+  se2w_spec_get_tables(spec, D, w1d, edx, edy);
+  if ((ierr = se2w_set_tables(sv->ctx, D, w1d, edx, edy))) { ierr = gpu_fail(ierr); goto bad; }
+  if ((ierr = se2w_set_cells(sv->ctx, spec->lambda, spec->mu))) { ierr = gpu_fail(ierr); goto bad; }
+  if ((ierr = se2w_spec_mass(spec, mass))) goto bad;
+  if ((ierr = se2w_set_mass(sv->ctx, mass))) { ierr = gpu_fail(ierr); goto bad; }
+  if (sv->ns > 0) {
+    elem = (int32_t *)malloc(sizeof(int32_t) * sv->ns);
+    vals = (double *)malloc(sizeof(double) * 2 * spec->npe * sv->ns);
+    if (!elem || !vals) { ierr = host_fail(SE2W_HOST_ERR_MEM, "out of memory"); goto bad; }
+    for (k = 0; k < sv->ns; k++)
+      if ((ierr = se2w_source_get(sv->src[k], &elem[k], NULL, &vals[(size_t)2 * spec->npe * k]))) goto bad;
+    if ((ierr = se2w_set_sources(sv->ctx, sv->ns, elem, vals))) { ierr = gpu_fail(ierr); goto bad; }
+  }
+  if (recv && recv->nr > 0) {
+    if ((ierr = se2w_set_receivers(sv->ctx, recv->nr, recv->elem, recv->gll))) { ierr = gpu_fail(ierr); goto bad; }
+  }
+  free(edx); free(edy); free(mass); free(elem); free(vals);
+  *out = sv;
+  return 0;
+bad:
+  free(edx); free(edy); free(mass); free(elem); free(vals);
+  se2w_solver_destroy(&sv);
+  return ierr;
+}
+
+int se2w_solver_destroy(SE2WSolver *sp)
+{
+  if (!sp || !*sp) return 0;
+  if ((*sp)->ctx) se2w_destroy((*sp)->ctx);
+  free((*sp)->src); free((*sp)->stf);
+  free((*sp)->times); free((*sp)->trace); free((*sp)->psi);
+  free(*sp);
+  *sp = NULL;
+  return 0;
+}
+
+se2w_ctx *se2w_solver_context(SE2WSolver sv) { return sv ? sv->ctx : NULL; }
+double se2w_solver_time(SE2WSolver sv) { return sv ? sv->time : 0.0; }
+int se2w_solver_steps_done(SE2WSolver sv) { return sv ? sv->steps : 0; }
+
+static int grow(double **p, size_t *cap, size_t n)
+{
+  if (n <= *cap) return 0;
+  free(*p);
+  *p = (double *)malloc(sizeof(double) * n);
+  if (!*p) { *cap = 0; return host_fail(SE2W_HOST_ERR_MEM, "out of memory"); }
+  *cap = n;
+  return 0;
+}
+
+int se2w_solver_advance(SE2WSolver sv, int nsteps, double dt)
+{
+  const int nr = sv && sv->recv ? sv->recv->nr : 0;
+  double t;
+  int k, s, ierr;
+  if (!sv || nsteps < 1) return host_fail(SE2W_HOST_ERR_ARG, "bad advance arguments");
+  if ((ierr = grow(&sv->times, &sv->times_cap, (size_t)nsteps))) return ierr;
+  if (sv->ns > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ns))) return ierr;
+  if (nr > 0 && (ierr = grow(&sv->trace, &sv->trace_cap, (size_t)nsteps * nr * 7))) return ierr;
+  t = sv->time;
+  for (k = 0; k < nsteps; k++) {
+    t = t + dt;
+    sv->times[k] = t;
+    for (s = 0; s < sv->ns; s++)
+      if ((ierr = se2w_stf_evaluate(sv->stf[s], t, &sv->psi[(size_t)k * sv->ns + s]))) return ierr;
+  }
+  if ((ierr = se2w_run(sv->ctx, nsteps, dt, sv->psi))) return gpu_fail(ierr);
+  if (nr > 0 && (ierr = se2w_get_receivers(sv->ctx, sv->trace))) return gpu_fail(ierr);
+  sv->time = t;
+  sv->steps += nsteps;
+  sv->last_n = nsteps;
+  return 0;
+}
+
+int se2w_solver_get_trace(SE2WSolver sv, double *times, double *trace)
+{
+  const int nr = sv && sv->recv ? sv->recv->nr : 0;
+  if (!sv) return host_fail(SE2W_HOST_ERR_ARG, "null solver");
+  if (times && sv->last_n) memcpy(times, sv->times, sizeof(double) * sv->last_n);
+  if (trace && nr > 0 && sv->last_n) memcpy(trace, sv->trace, sizeof(double) * (size_t)sv->last_n * nr * 7);
+  return 0;
+}
