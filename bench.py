@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the explicit spectral-element time step on one MI355X.
+
+Workload: 512 x 512 elements of degree 4 (2049^2 GLL nodes), per-cell random
+material (Vp ~ U[4000,6000], Vs = Vp/sqrt(3), rho ~ U[2500,3000], seed 2),
+two Ricker sources (15 Hz), 16 receivers, dt = 0.2 CFL.  A step is one
+explicit Verlet step of the whole mesh with receiver sampling; the inputs
+are resident in HBM when the timed region starts.
+
+Prints one JSON line: GLL node-updates/s of the FP32 (compensated) path,
+with the FP64 path, the accuracy of FP32 against FP64 over the full
+2000-step run, the roofline of the dominant kernel and the reference
+algorithm timed on the host CPU (oracle/).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "GLL node-updates/s (FP32 vs FP64) at 512² p4; ensemble solves/s; surrogate err"
+# algorithmic HBM bytes per node and step of the fused step kernel:
+# read u, v, 1/M (or M); write u, v  (+ compensation read/write)
+BYTES_PER_NODE = {0: 2 * 8 + 2 * 8 + 8 + 2 * 8 + 2 * 8,   # FP64: 72
+                  1: 2 * 4 + 2 * 4 + 4 + 2 * 4 + 2 * 4,   # FP32: 36
+                  2: 2 * 4 + 2 * 4 + 4 + 2 * 4 + 2 * 4 + 2 * 2 * 4}  # FP32 + Kahan: 52
+CANONICAL_B_NODE = {0: 152, 1: 76, 2: 92}  # two-pass scheme bytes/node/step
+
+
+def rel_l2(a, b):
+    n = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
+
+
+def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, kernel=None):
+    b = SC.build(sc, precision=prec)
+    sv = b.solver
+    if kernel is not None:
+        sv.ctx.set_kernel(kernel)
+    if warmup > 0:
+        sv.advance(warmup, b.dt)
+    t0 = time.perf_counter()
+    sv.advance(steps, b.dt)
+    wall = time.perf_counter() - t0
+    run_ms, _ = sv.ctx.timing()
+    # per-launch kernel time (HIP events on the library's stream)
+    sv.ctx.set_profiling(True)
+    sv.advance(profile_steps, b.dt)
+    _, kern = sv.ctx.timing()
+    sv.ctx.set_profiling(False)
+    sv.close()
+    out = {"wall_s": wall, "event_ms": run_ms, "kernel_ms": float(kern[0]), "dt": b.dt}
+    if full_steps > 0:
+        bf = SC.build(sc, precision=prec)
+        if kernel is not None:
+            bf.solver.ctx.set_kernel(kernel)
+        _, tr = bf.solver.advance(full_steps, bf.dt)
+        out["trace"] = tr
+        bf.solver.close()
+    return out
+
+
+def cpu_baseline(sc, dt, seconds_target=15.0):
+    """The reference algorithm (dense FP64 element loop, oracle/) on the host."""
+    from oracle import oracle as O
+    cores = min(os.cpu_count() or 1, 16)
+    pr = O.Problem(sc, dt=dt, nthreads=cores, nt=1)
+    t0 = time.perf_counter()
+    pr.run(max_steps=1, want_state=False)
+    one = time.perf_counter() - t0
+    nsteps = max(1, min(200, int(seconds_target / max(one, 1e-3))))
+    pr = O.Problem(sc, dt=dt, nthreads=cores, nt=nsteps)
+    t0 = time.perf_counter()
+    pr.run(max_steps=nsteps, want_state=False)
+    el = time.perf_counter() - t0
+    return {"value": sc.nnodes * nsteps / el, "unit": "node-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{nsteps} steps of the 512x512 p=4 workload (FP64 dense reference element loop, "
+                      f"oracle/ref_cpu.c, {cores} OpenMP threads), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--full-steps", type=int, default=2000, help="length of the accuracy runs (0: skip)")
+    ap.add_argument("--profile-steps", type=int, default=100)
+    ap.add_argument("--mx", type=int, default=512)
+    ap.add_argument("--p", type=int, default=4)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precisions", default="fp32k,fp32,fp64")
+    ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
+                    help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
+                                   "(see DESIGN.md), so --gpus must be 1"}))
+        sys.exit(2)
+    import se2dr_fork_amd as S
+    from se2dr_fork_amd import scenarios as SC
+
+    sc = SC.config2(nt=max(args.full_steps, 1), mx=args.mx, p=args.p)
+    precs = [S.PRECISIONS[x] for x in args.precisions.split(",")]
+    res = {}
+    for prec in precs:
+        kern = None if (args.kernel == "auto" or prec == S.FP64) else {"lds": S.KERNEL_LDS, "reg": S.KERNEL_REG}[args.kernel]
+        res[prec] = run_precision(S, SC, sc, prec, args.steps, args.warmup, args.full_steps, args.profile_steps, kern)
+    head = precs[0]
+    nodes = sc.nnodes
+    rates = {p: nodes * args.steps / r["wall_s"] for p, r in res.items()}
+    value = rates[head]
+    ms_step = 1e3 * res[head]["wall_s"] / args.steps
+    kern_ms = res[head]["kernel_ms"]
+    achieved = BYTES_PER_NODE[head] * nodes / (kern_ms * 1e-3) / 1e9 if kern_ms > 0 else None
+    errs = {}
+    if S.FP64 in res and args.full_steps > 0:
+        ref = res[S.FP64]["trace"]
+        for p in precs:
+            if p != S.FP64:
+                errs[p] = rel_l2(res[p]["trace"], ref)
+    names = {0: "fp64", 1: "fp32", 2: "fp32k"}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "node-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {0: "f64", 1: "f32", 2: "f32"}[head],
+        "data": "synthetic",
+        "config": {"workload": f"se2wave explicit SEM step, {args.mx}x{args.mx} elements, p={args.p} "
+                               f"({nodes} GLL nodes), per-cell random material, 2 Ricker sources, "
+                               f"16 receivers, {names[head]} arithmetic, fused step kernel",
+                   "mx": args.mx, "my": args.mx, "p": args.p, "nodes": nodes},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
+                     "kernel": "tile_kernel (fused element operator + Verlet update)",
+                     "bytes_per_node": BYTES_PER_NODE[head], "kernel_ms": kern_ms},
+        "node_updates_per_s": {names[p]: r for p, r in rates.items()},
+        "canonical_roofline_frac": {names[p]: rates[p] * CANONICAL_B_NODE[p] / (HBM_PEAK_GBS * 1e9) for p in rates},
+        "kernel_ms": {names[p]: r["kernel_ms"] for p, r in res.items()},
+        "receiver_rel_l2_vs_fp64": {names[p]: e for p, e in errs.items()},
+        "full_steps": args.full_steps,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"])
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

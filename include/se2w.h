@@ -115,7 +115,28 @@ int se2w_norms(se2w_ctx *ctx, double out[6]);
 int se2w_set_profiling(se2w_ctx *ctx, int32_t on);
 int se2w_get_timing(se2w_ctx *ctx, double *run_ms, double kernel_ms[2]);
 
-/* Tile shape (elements per workgroup) used by the step kernels. */
+/* Element-kernel variants: SE2W_KERNEL_LDS, or SE2W_KERNEL_REG + k for the
+ * k-th schedule (workgroup shape, register cap) of the register-blocked FP32
+ * kernel (2 <= p <= 6); se2w_kernel_count() variants in all. */
+#define SE2W_KERNEL_LDS 0 /* LDS-staged tile kernel (all modes, 1 <= p <= 8) */
+#define SE2W_KERNEL_REG 1 /* register-blocked FP32 kernel, first schedule */
+
+/* Selects / reports the element-kernel variant (no counterpart in the
+ * reference: a schedule choice of this implementation, used by the
+ * autotuner).  Unavailable variants return SE2W_ERR_UNSUPPORTED. */
+int se2w_set_kernel(se2w_ctx *ctx, int32_t variant);
+int se2w_get_kernel(se2w_ctx *ctx, int32_t *variant);
+int se2w_kernel_count(void);
+
+/* Autotuner (no counterpart in the reference): runs `reps` timed trials of
+ * `steps` steps for every element-kernel variant built for this context's
+ * degree, precision and schedule, keeps the fastest (median kernel time) and
+ * reports it.  The state u, v, a is restored afterwards; receiver rows of the
+ * previous se2w_run are discarded.  The choice is cached per (p, precision,
+ * schedule, size bucket) for the rest of the process. */
+int se2w_autotune(se2w_ctx *ctx, int32_t steps, int32_t reps, int32_t *best_variant, double *best_ms);
+
+/* Tile shape (owned elements per workgroup) used by the step kernels. */
 int se2w_get_tile(se2w_ctx *ctx, int32_t *tex, int32_t *tey);
 
 int se2w_destroy(se2w_ctx *ctx);

@@ -21,6 +21,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 FP64, FP32, FP32_KAHAN = 0, 1, 2
+KERNEL_LDS, KERNEL_REG = 0, 1
 SCHED_FUSED, SCHED_SPLIT = 0, 1
 SOURCE_FORCE, SOURCE_MOMENT = 0, 1
 IMPL_POINTWISE, IMPL_NEAREST_QPOINT, IMPL_DEFAULT = 0, 1, 3
@@ -78,6 +79,9 @@ def _declare(hip, host):
     hip.se2w_set_profiling.argtypes = [v, i32]
     hip.se2w_get_timing.argtypes = [v, _dp, _dp]
     hip.se2w_get_tile.argtypes = [v, _ip, _ip]
+    hip.se2w_set_kernel.argtypes = [v, i32]
+    hip.se2w_get_kernel.argtypes = [v, _ip]
+    hip.se2w_autotune.argtypes = [v, i32, i32, _ip, _dp]
     hip.se2w_destroy.argtypes = [v]
     host.se2w_host_last_error.restype = C.c_char_p
     host.se2w_gll.argtypes = [C.c_int, _dp, _dp]
@@ -480,6 +484,19 @@ class Context:
         k = np.zeros(2)
         _gchk(self._hip.se2w_get_timing(self._handle, C.byref(ms), _ptr(k)))
         return ms.value, k
+
+    def set_kernel(self, variant: int):
+        _gchk(self._hip.se2w_set_kernel(self._handle, variant))
+
+    def autotune(self, steps=20, reps=3):
+        best, ms = C.c_int32(), C.c_double()
+        _gchk(self._hip.se2w_autotune(self._handle, steps, reps, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
+    def kernel(self) -> int:
+        v = C.c_int32()
+        _gchk(self._hip.se2w_get_kernel(self._handle, C.byref(v)))
+        return v.value
 
     def tile(self):
         a, b = C.c_int32(), C.c_int32()

@@ -3,6 +3,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -11,6 +14,7 @@
 
 #include "se2w.h"
 #include "se2w_kernels.hpp"
+#include "se2w_kernels_reg.hpp"
 
 using namespace se2w;
 
@@ -114,7 +118,7 @@ size_t real_size(int mode) { return mode == SE2W_FP64 ? sizeof(double) : sizeof(
 
 struct se2w_ctx {
   int mx = 0, my = 0, p = 0, nb = 0, npe = 0, nx = 0, ny = 0, pitch = 0;
-  int mode = 0, schedule = 0, device = 0;
+  int mode = 0, schedule = 0, device = 0, variant = 0;
   int tex = 0, tey = 0, tiles_x = 0, tiles_y = 0, ntiles = 0, tiles_per_xcd = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -128,6 +132,7 @@ struct se2w_ctx {
   std::vector<double> hD, hdx, hdy;
   // sources
   int ns = 0;
+  std::vector<int> h_src_elem;
   int *d_src_elem = nullptr, *d_tsrc_ptr = nullptr, *d_tsrc_idx = nullptr;
   void *d_src_vals = nullptr;
   void *d_psi = nullptr;
@@ -181,7 +186,20 @@ int upload_real(se2w_ctx *c, void **dst, const double *src, size_t n) {
   return upload<float>(dst, tmp.data(), n, c->stream);
 }
 
-// Tile shapes of the kernels compiled for (p, precision).
+// Register-blocked kernel availability: FP32 modes, 2 <= p <= 6.
+bool reg_available(int p, int mode) { return mode != SE2W_FP64 && p >= 2 && p <= 6; }
+
+// variant = SE2W_KERNEL_REG + schedule index into kRegSchedules
+int reg_schedule_of(int variant) { return variant - SE2W_KERNEL_REG; }
+bool is_reg_variant(int variant) { return variant >= SE2W_KERNEL_REG && variant < SE2W_KERNEL_REG + kNumRegSchedules; }
+
+void reg_shape(int variant, int &tex, int &tey) {
+  const RegSchedule &r = kRegSchedules[reg_schedule_of(variant)];
+  tex = r.ex - 1;
+  tey = r.ey - 1;
+}
+
+// Tile shapes of the LDS-staged kernels compiled for (p, precision).
 void tile_shape(int p, int mode, int &tex, int &tey) {
 #define CASE(P_)                                                                                     \
   case P_:                                                                                           \
@@ -220,6 +238,15 @@ void build_tile_lists(const se2w_ctx *c, const std::vector<int> &elem, std::vect
   }
 }
 
+void set_tiling(se2w_ctx *c) {
+  if (is_reg_variant(c->variant)) reg_shape(c->variant, c->tex, c->tey);
+  else tile_shape(c->p, c->mode, c->tex, c->tey);
+  c->tiles_x = (c->mx + c->tex - 1) / c->tex;
+  c->tiles_y = (c->my + c->tey - 1) / c->tey;
+  c->ntiles = c->tiles_x * c->tiles_y;
+  c->tiles_per_xcd = (c->ntiles + 7) / 8;
+}
+
 template <int MODE>
 StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double dt, int last) {
   using R = typename RealOf<MODE>::T;
@@ -228,6 +255,7 @@ StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double d
   memset(&p, 0, sizeof(p));
   p.mx = c->mx; p.my = c->my; p.nx = c->nx; p.ny = c->ny; p.pitch = c->pitch;
   p.tiles_x = c->tiles_x; p.tiles_y = c->tiles_y; p.tiles_per_xcd = c->tiles_per_xcd;
+  p.tex = c->tex; p.tey = c->tey;
   p.u_in = (const R2 *)c->U[c->cur];
   p.u_out = (R2 *)c->U[c->cur ^ 1];
   p.v = (R2 *)c->V;
@@ -255,15 +283,50 @@ StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double d
   return p;
 }
 
+// Schedules instantiated for (p, fused): all of them for the degree of the
+// headline workload, the first three elsewhere, the first one for the split
+// schedule's element kernel.
+constexpr bool reg_schedule_built(int P, int S, bool fused) { return fused ? (P == 4 || S < 3) : S == 0; }
+
+bool reg_variant_built(int p, int variant, int schedule) {
+  return is_reg_variant(variant) && reg_schedule_built(p, reg_schedule_of(variant), schedule == SE2W_SCHED_FUSED);
+}
+
+template <int P, int MODE, bool FUSED, int S>
+bool launch_reg_s(se2w_ctx *c, const StepParams<float> &prm) {
+  if constexpr (S < kNumRegSchedules) {
+    if (reg_schedule_built(P, S, FUSED) && reg_schedule_of(c->variant) == S) {
+      if constexpr (reg_schedule_built(P, S, FUSED)) {
+        constexpr RegSchedule r = kRegSchedules[S];
+        hipLaunchKernelGGL((reg_kernel<P, MODE, FUSED, r.ex, r.ey, r.waves>), dim3(8 * c->tiles_per_xcd),
+                           dim3(r.ex * r.ey), 0, c->stream, prm);
+        return true;
+      }
+    }
+    return launch_reg_s<P, MODE, FUSED, S + 1>(c, prm);
+  }
+  return false;
+}
+
+// Element (+ update when fused) kernel of the context's variant.
+template <int P, int MODE, bool FUSED>
+void launch_element_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
+  const dim3 grid(8 * c->tiles_per_xcd);
+  if constexpr (MODE != MODE_F64 && P >= 2 && P <= 6) {
+    if (is_reg_variant(c->variant) && launch_reg_s<P, MODE, FUSED, 0>(c, prm)) return;
+  }
+  hipLaunchKernelGGL((tile_kernel<P, MODE, FUSED>), grid, dim3(kThreads), 0, c->stream, prm);
+}
+
 template <int P, int MODE>
 void launch_step_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm, hipEvent_t *evs) {
   const dim3 grid(8 * c->tiles_per_xcd), block(kThreads);
   if (evs) hipEventRecord(evs[0], c->stream);
   if (c->schedule == SE2W_SCHED_FUSED) {
-    hipLaunchKernelGGL((tile_kernel<P, MODE, true>), grid, block, 0, c->stream, prm);
+    launch_element_p<P, MODE, true>(c, prm);
     if (evs) hipEventRecord(evs[1], c->stream);
   } else {
-    hipLaunchKernelGGL((tile_kernel<P, MODE, false>), grid, block, 0, c->stream, prm);
+    launch_element_p<P, MODE, false>(c, prm);
     if (evs) hipEventRecord(evs[1], c->stream);
     hipLaunchKernelGGL((update_kernel<P, MODE>), grid, block, 0, c->stream, prm);
     if (evs) hipEventRecord(evs[2], c->stream);
@@ -287,7 +350,7 @@ void launch_step(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm, h
 
 template <int P, int MODE>
 void launch_element_only_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
-  hipLaunchKernelGGL((tile_kernel<P, MODE, false>), dim3(8 * c->tiles_per_xcd), dim3(kThreads), 0, c->stream, prm);
+  launch_element_p<P, MODE, false>(c, prm);
 }
 
 template <int MODE>
@@ -391,11 +454,8 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
   c->mode = desc->precision;
   c->schedule = desc->schedule;
   c->device = desc->device;
-  tile_shape(c->p, c->mode, c->tex, c->tey);
-  c->tiles_x = (c->mx + c->tex - 1) / c->tex;
-  c->tiles_y = (c->my + c->tey - 1) / c->tey;
-  c->ntiles = c->tiles_x * c->tiles_y;
-  c->tiles_per_xcd = (c->ntiles + 7) / 8;
+  c->variant = reg_available(c->p, c->mode) ? SE2W_KERNEL_REG : SE2W_KERNEL_LDS;
+  set_tiling(c);
   int ierr = SE2W_OK;
   hipError_t he = hipSetDevice(c->device);
   if (he != hipSuccess) { ierr = fail(SE2W_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); goto bad; }
@@ -528,6 +588,7 @@ int se2w_set_sources(se2w_ctx *c, int32_t ns, const int32_t *elem, const double 
     if (el[s] < 0 || el[s] >= ne) return fail(SE2W_ERR_ARG, "source %d: element %d out of range", s, el[s]);
   std::vector<int> ptr, idx;
   build_tile_lists(c, el, ptr, idx);
+  c->h_src_elem = el;
   int ierr;
   if ((ierr = upload<int>((void **)&c->d_src_elem, el.data(), el.size(), c->stream)) ||
       (ierr = upload<int>((void **)&c->d_tsrc_ptr, ptr.data(), ptr.size(), c->stream)) ||
@@ -776,6 +837,126 @@ int se2w_get_timing(se2w_ctx *c, double *run_ms, double kernel_ms[2]) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   if (run_ms) *run_ms = c->last_ms;
   if (kernel_ms) { kernel_ms[0] = c->kern_ms[0]; kernel_ms[1] = c->kern_ms[1]; }
+  return SE2W_OK;
+}
+
+int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant)) return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
+  if (is_reg_variant(variant) && !reg_available(c->p, c->mode))
+    return fail(SE2W_ERR_UNSUPPORTED, "register-blocked kernel needs FP32 and 2 <= p <= 6");
+  if (is_reg_variant(variant) && !reg_variant_built(c->p, variant, c->schedule))
+    return fail(SE2W_ERR_UNSUPPORTED, "kernel variant %d is not built for p = %d and this schedule", variant, c->p);
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->variant = variant;
+  set_tiling(c);
+  std::vector<int> ptr, idx;
+  int ierr;
+  build_tile_lists(c, c->h_src_elem, ptr, idx);
+  if ((ierr = upload<int>((void **)&c->d_tsrc_ptr, ptr.data(), ptr.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_tsrc_idx, idx.data(), idx.size(), c->stream)))
+    return ierr;
+  build_tile_lists(c, c->h_rec_elem, ptr, idx);
+  if ((ierr = upload<int>((void **)&c->d_trec_ptr, ptr.data(), ptr.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_trec_idx, idx.data(), idx.size(), c->stream)))
+    return ierr;
+  return SE2W_OK;
+}
+
+int se2w_kernel_count(void) { return SE2W_KERNEL_REG + kNumRegSchedules; }
+
+// Autotuner: times every variant built for the context's (p, precision,
+// schedule) on the context's own mesh and keeps the fastest.  The state is
+// snapshotted and restored around the trials, so tuning does not perturb the
+// simulation.  Choices are cached per (p, precision, schedule, size bucket)
+// for the life of the process.
+namespace {
+std::mutex g_tune_mu;
+std::map<std::tuple<int, int, int, int, int>, int> g_tune_cache;
+int size_bucket(int n) {
+  int b = 0;
+  while (n > 1) { n >>= 1; b++; }
+  return b;
+}
+}  // namespace
+
+int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_variant, double *best_ms) {
+  if (!c || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
+  if (!c->have_tables || !c->have_cells || !c->have_mass) return fail(SE2W_ERR_STATE, "set tables, cells and mass first");
+  HIPCHK(hipSetDevice(c->device));
+  const auto key = std::make_tuple(c->p, c->mode, c->schedule, size_bucket(c->mx), size_bucket(c->my));
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      int ierr = se2w_set_kernel(c, it->second);
+      if (ierr) return ierr;
+      if (best_variant) *best_variant = it->second;
+      if (best_ms) *best_ms = -1.0;
+      return SE2W_OK;
+    }
+  }
+  std::vector<int> cands = {SE2W_KERNEL_LDS};
+  if (reg_available(c->p, c->mode))
+    for (int k = 0; k < kNumRegSchedules; k++)
+      if (reg_variant_built(c->p, SE2W_KERNEL_REG + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG + k);
+  // snapshot the state
+  const size_t field = (size_t)c->pitch * c->ny * 2 * real_size(c->mode);
+  std::vector<void *> snap;
+  void *srcs[4] = {c->U[c->cur], c->V, c->A, c->C[c->cur]};
+  for (void *src : srcs) {
+    void *d = nullptr;
+    if (src) {
+      int ierr = dev_alloc(&d, field);
+      if (ierr) { for (void *x : snap) hipFree(x); return ierr; }
+      HIPCHK(hipMemcpyAsync(d, src, field, hipMemcpyDeviceToDevice, c->stream));
+    }
+    snap.push_back(d);
+  }
+  const int cur0 = c->cur, variant0 = c->variant, ns0 = c->ns, nr0 = c->nr;
+  std::vector<double> zeros((size_t)steps * std::max(c->ns, 1), 0.0);
+  int best = variant0;
+  double best_t = 1e300;
+  const bool prof0 = c->profiling;
+  c->profiling = true;
+  for (int v : cands) {
+    if (se2w_set_kernel(c, v)) continue;
+    std::vector<double> t;
+    int ierr = SE2W_OK;
+    for (int r = 0; r < reps + 1 && !ierr; r++) {  // first repetition is a warm-up
+      ierr = se2w_run(c, steps, 1.0e-30, zeros.data());
+      if (!ierr && r > 0) t.push_back(c->kern_ms[0] + c->kern_ms[1]);
+    }
+    if (ierr || t.empty()) continue;
+    std::sort(t.begin(), t.end());
+    const double med = t[t.size() / 2];
+    if (med < best_t) { best_t = med; best = v; }
+  }
+  c->profiling = prof0;
+  // restore the state and the receiver bookkeeping
+  c->cur = cur0;
+  for (int f = 0; f < 4; f++)
+    if (snap[f]) HIPCHK(hipMemcpyAsync(srcs[f], snap[f], field, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (void *x : snap) if (x) hipFree(x);
+  c->ns = ns0;
+  c->nr = nr0;
+  c->rec_steps = 0;
+  int ierr = se2w_set_kernel(c, best);
+  if (ierr) return ierr;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  if (best_variant) *best_variant = best;
+  if (best_ms) *best_ms = best_t;
+  return SE2W_OK;
+}
+
+int se2w_get_kernel(se2w_ctx *c, int32_t *variant) {
+  if (!c || !variant) return fail(SE2W_ERR_ARG, "null argument");
+  *variant = c->variant;
   return SE2W_OK;
 }
 

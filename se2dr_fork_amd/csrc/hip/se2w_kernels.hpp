@@ -64,6 +64,7 @@ template <typename R> struct StepParams {
   using R2 = typename V2<R>::T;
   int mx, my, nx, ny, pitch;
   int tiles_x, tiles_y, tiles_per_xcd;
+  int tex, tey;    // owned elements per tile of the kernel in use
   const R2 *u_in;  // displacement read by the element operator
   R2 *u_out;       // displacement written by the update (may alias u_in in split mode)
   R2 *v;           // velocity, updated in place
@@ -228,21 +229,92 @@ __device__ __forceinline__ void force_f32(const float *Sxx, const float *Syy, co
   fy = -(iJ00 * ry + iJ11 * cy);
 }
 
+// Receiver sampling: every node of a receiver's element records (u, v, a).
+template <int P, typename R>
+__device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int tile, int gx, int gy, R ux, R uy,
+                                                 R vx, R vy, R ax, R ay) {
+  constexpr int NB = P + 1;
+  const int r0 = prm.tile_rec_ptr[tile], r1 = prm.tile_rec_ptr[tile + 1];
+  for (int t = r0; t < r1; t++) {
+    const int r = prm.tile_rec_idx[t];
+    const int e = prm.rec_elem[r];
+    const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
+    if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
+      R *o = prm.rec + (((long)prm.step * prm.nr + r) * (NB * NB) + li + lj * NB) * 6;
+      o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy; o[4] = ax; o[5] = ay;
+    }
+  }
+}
+
+// FP64: a = f / M; v += (0.5 dt) a; record; u += dt v; u += (0.5 dt^2) a;
+// v += (0.5 dt) a -- each operation rounded as in the reference's AXPYs.
+template <int P>
+__device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, int tile, int gx, int gy, long idx,
+                                                double fx, double fy, double ux, double uy, double2 vv, double mass) {
+#pragma clang fp contract(off)
+  const double ax = fx / mass, ay = fy / mass;
+  const double vx = vv.x + prm.hdt * ax, vy = vv.y + prm.hdt * ay;
+  record_receivers<P, double>(prm, tile, gx, gy, ux, uy, vx, vy, ax, ay);
+  if (prm.last) {
+    prm.u_out[idx] = double2{ux, uy};
+    prm.v[idx] = double2{vx, vy};
+    prm.a[idx] = double2{ax, ay};
+    return;
+  }
+  double nux = ux + prm.dt * vx, nuy = uy + prm.dt * vy;
+  nux = nux + prm.hdt2 * ax;
+  nuy = nuy + prm.hdt2 * ay;
+  prm.u_out[idx] = double2{nux, nuy};
+  prm.v[idx] = double2{vx + prm.hdt * ax, vy + prm.hdt * ay};
+}
+
+// FP32: same sequence with fused multiply-adds; MODE_F32K accumulates the
+// displacement with a compensated (Kahan) sum, one per reference AXPY.
+template <int P, int MODE>
+__device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, int tile, int gx, int gy, long idx,
+                                                float fx, float fy, float ux, float uy, float2 vv, float minv,
+                                                float2 cc) {
+  const float ax = fx * minv, ay = fy * minv;
+  const float vx = fmaf(prm.hdt, ax, vv.x), vy = fmaf(prm.hdt, ay, vv.y);
+  record_receivers<P, float>(prm, tile, gx, gy, ux, uy, vx, vy, ax, ay);
+  if (prm.last) {
+    prm.u_out[idx] = float2{ux, uy};
+    prm.v[idx] = float2{vx, vy};
+    prm.a[idx] = float2{ax, ay};
+    if constexpr (MODE == MODE_F32K) prm.c_out[idx] = cc;
+    return;
+  }
+  float nux, nuy;
+  if constexpr (MODE == MODE_F32K) {
+    float2 c = cc;
+    float y, t;
+    y = fmaf(prm.dt, vx, -c.x); t = ux + y; c.x = (t - ux) - y; nux = t;
+    y = fmaf(prm.dt, vy, -c.y); t = uy + y; c.y = (t - uy) - y; nuy = t;
+    const float m0 = nux, m1 = nuy;
+    y = fmaf(prm.hdt2, ax, -c.x); t = m0 + y; c.x = (t - m0) - y; nux = t;
+    y = fmaf(prm.hdt2, ay, -c.y); t = m1 + y; c.y = (t - m1) - y; nuy = t;
+    prm.c_out[idx] = c;
+  } else {
+    nux = fmaf(prm.hdt2, ax, fmaf(prm.dt, vx, ux));
+    nuy = fmaf(prm.hdt2, ay, fmaf(prm.dt, vy, uy));
+  }
+  prm.u_out[idx] = float2{nux, nuy};
+  prm.v[idx] = float2{fmaf(prm.hdt, ax, vx), fmaf(prm.hdt, ay, vy)};
+}
+
 // ---------------------------------------------------------------------------
-// Phase D: corrector of step n+1, receiver sampling, predictor of step n+2.
-// Sequence per node (se2wave.c:4739-4745 then 4726-4730):
-//   f = f + g;  a = f / M;  v = v + (0.5 dt) a;  [record u, v, a]
-//   u = u + dt v;  u = u + (0.5 dt^2) a;  v = v + (0.5 dt) a.
+// Node update with the nodal v, mass and compensation already loaded.
 // ---------------------------------------------------------------------------
 template <int P, int MODE>
-__device__ __forceinline__ void node_update(const StepParams<typename RealOf<MODE>::T> &prm, int tile, int gx,
-                                            int gy, typename RealOf<MODE>::T fx, typename RealOf<MODE>::T fy,
-                                            typename RealOf<MODE>::T ux, typename RealOf<MODE>::T uy) {
+__device__ __forceinline__ void node_update_core(const StepParams<typename RealOf<MODE>::T> &prm, int tile, int gx,
+                                                 int gy, long idx, typename RealOf<MODE>::T fx,
+                                                 typename RealOf<MODE>::T fy, typename RealOf<MODE>::T ux,
+                                                 typename RealOf<MODE>::T uy, typename V2<typename RealOf<MODE>::T>::T vv,
+                                                 typename RealOf<MODE>::T mass,
+                                                 typename V2<typename RealOf<MODE>::T>::T cc) {
 #pragma clang fp contract(off)
   using R = typename RealOf<MODE>::T;
-  using R2 = typename V2<R>::T;
   constexpr int NB = P + 1;
-  const long idx = (long)gy * prm.pitch + gx;
   // g = sum_s psi_s * vals_s in source order, then f = f + g (3476-3495, 4739)
   {
     const int s0 = prm.tile_src_ptr[tile], s1 = prm.tile_src_ptr[tile + 1];
@@ -267,58 +339,27 @@ __device__ __forceinline__ void node_update(const StepParams<typename RealOf<MOD
       }
     }
   }
-  R ax, ay;
   if constexpr (MODE == MODE_F64) {
-    const R M = prm.mass[idx];
-    ax = fx / M;
-    ay = fy / M;
+    node_finish_f64<P>(prm, tile, gx, gy, idx, fx, fy, ux, uy, vv, mass);
   } else {
-    const R mi = prm.mass[idx];
-    ax = fx * mi;
-    ay = fy * mi;
+    node_finish_f32<P, MODE>(prm, tile, gx, gy, idx, fx, fy, ux, uy, vv, mass, cc);
   }
-  R2 vv = prm.v[idx];
-  R vx = vv.x + prm.hdt * ax;
-  R vy = vv.y + prm.hdt * ay;
-  // receivers: every node of a receiver's element records (u, v, a)
-  {
-    const int r0 = prm.tile_rec_ptr[tile], r1 = prm.tile_rec_ptr[tile + 1];
-    for (int t = r0; t < r1; t++) {
-      const int r = prm.tile_rec_idx[t];
-      const int e = prm.rec_elem[r];
-      const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
-      if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
-        R *o = prm.rec + (((long)prm.step * prm.nr + r) * (NB * NB) + li + lj * NB) * 6;
-        o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy; o[4] = ax; o[5] = ay;
-      }
-    }
-  }
-  if (prm.last) {
-    prm.u_out[idx] = R2{ux, uy};
-    prm.v[idx] = R2{vx, vy};
-    prm.a[idx] = R2{ax, ay};
-    if constexpr (MODE == MODE_F32K) prm.c_out[idx] = prm.c_in[idx];
-    return;
-  }
-  R nux, nuy;
-  if constexpr (MODE == MODE_F32K) {
-    // two compensated additions mirroring the reference's two AXPYs
-    R2 c = prm.c_in[idx];
-    R y, t;
-    y = prm.dt * vx - c.x; t = ux + y; c.x = (t - ux) - y; nux = t;
-    y = prm.dt * vy - c.y; t = uy + y; c.y = (t - uy) - y; nuy = t;
-    R mx_ = nux, my_ = nuy;
-    y = prm.hdt2 * ax - c.x; t = mx_ + y; c.x = (t - mx_) - y; nux = t;
-    y = prm.hdt2 * ay - c.y; t = my_ + y; c.y = (t - my_) - y; nuy = t;
-    prm.c_out[idx] = c;
-  } else {
-    nux = ux + prm.dt * vx;
-    nuy = uy + prm.dt * vy;
-    nux = nux + prm.hdt2 * ax;
-    nuy = nuy + prm.hdt2 * ay;
-  }
-  prm.u_out[idx] = R2{nux, nuy};
-  prm.v[idx] = R2{vx + prm.hdt * ax, vy + prm.hdt * ay};
+}
+
+// Phase D: corrector of step n+1, receiver sampling, predictor of step n+2.
+// Sequence per node (se2wave.c:4739-4745 then 4726-4730):
+//   f = f + g;  a = f / M;  v = v + (0.5 dt) a;  [record u, v, a]
+//   u = u + dt v;  u = u + (0.5 dt^2) a;  v = v + (0.5 dt) a.
+template <int P, int MODE>
+__device__ __forceinline__ void node_update(const StepParams<typename RealOf<MODE>::T> &prm, int tile, int gx,
+                                            int gy, typename RealOf<MODE>::T fx, typename RealOf<MODE>::T fy,
+                                            typename RealOf<MODE>::T ux, typename RealOf<MODE>::T uy) {
+  using R = typename RealOf<MODE>::T;
+  using R2 = typename V2<R>::T;
+  const long idx = (long)gy * prm.pitch + gx;
+  R2 cc = R2{R(0), R(0)};
+  if constexpr (MODE == MODE_F32K) cc = prm.c_in[idx];
+  node_update_core<P, MODE>(prm, tile, gx, gy, idx, fx, fy, ux, uy, prm.v[idx], prm.mass[idx], cc);
 }
 
 // ---------------------------------------------------------------------------
@@ -448,7 +489,7 @@ template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void update_kernel(StepParams<typename RealOf<MODE>::T> prm) {
   using R = typename RealOf<MODE>::T;
   using R2 = typename V2<R>::T;
-  constexpr int TEX = TileShape<P, R>::TEX, TEY = TileShape<P, R>::TEY;
+  const int TEX = prm.tex, TEY = prm.tey;
   const int ntiles = prm.tiles_x * prm.tiles_y;
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= ntiles) return;

@@ -4,6 +4,7 @@
  * functions evaluated on the host at the accumulated time
  * (time = time + dt, 4724) and the receiver rows collected per advance.
  */
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -78,6 +79,17 @@ int se2w_solver_create(SE2WSpec spec, int precision, int schedule, int ns, SE2WS
     if ((ierr = se2w_set_receivers(sv->ctx, recv->nr, recv->elem, recv->gll))) { ierr = gpu_fail(ierr); goto bad; }
   }
   free(edx); free(edy); free(mass); free(elem); free(vals);
+  /* Pick the fastest element-kernel schedule for this mesh (SE2W_AUTOTUNE=0
+     keeps the default; small meshes keep it too, tuning them costs more than
+     it saves). */
+  {
+    const char *env = getenv("SE2W_AUTOTUNE");
+    const int want = env ? atoi(env) : (spec->nnodes >= 65536);
+    if (want) {
+      int32_t best;
+      if ((ierr = se2w_autotune(sv->ctx, 20, 3, &best, NULL))) { ierr = gpu_fail(ierr); goto bad; }
+    }
+  }
   *out = sv;
   return 0;
 bad:

@@ -1,0 +1,263 @@
+"""GPU parity: the HIP step through the C ABI against the FP64 CPU oracle.
+
+FP64 mode follows the reference's operation order with FMA contraction off,
+so it must be bit-identical to the oracle: receiver rows, u, v and a.  FP32
+modes are checked against FP64 within stated tolerances.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import se2dr_fork_amd as S
+from se2dr_fork_amd import scenarios as SC
+from conftest import GOLDEN, ROOT, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def gpu_run(sc, precision=S.FP64, schedule=S.SCHED_FUSED, nsteps=None, kernel=None, chunks=1):
+    b = SC.build(sc, precision=precision, schedule=schedule)
+    if kernel is not None:
+        b.solver.ctx.set_kernel(kernel)
+    n = nsteps or b.nsteps
+    times, rows = [], []
+    base, rem = divmod(n, chunks)
+    for c in range(chunks):
+        k = base + (1 if c < rem else 0)
+        t, r = b.solver.advance(k, b.dt)
+        times.append(t)
+        rows.append(r)
+    u, v, a = b.solver.ctx.get_state()
+    return b, np.concatenate(times), np.concatenate(rows), u, v, a
+
+
+def oracle_run(oracle, sc, b):
+    pr = oracle.Problem(sc, dt=b.dt, nt=b.nsteps)
+    return pr.run()
+
+
+def small_cases():
+    yield SC.small(mx=6, my=5, p=3, nt=40)
+    yield SC.small(mx=7, my=3, p=4, kind=S.SOURCE_FORCE, nt=35)
+    yield SC.demo(mx=8, p=2, nt=50, cfl_factor=0.5)
+    for p in [1, 2, 5, 6, 7, 8]:
+        sc = SC.small(mx=5, my=4, p=p, nt=25, seed=20 + p)
+        sc.use_nearest_internal_gll = p > 1
+        yield sc
+    yield SC.small(mx=33, my=17, p=4, nt=30, nsrc=3, nrec=9)  # several tiles, partial edge tiles
+
+
+@pytest.mark.parametrize("sc", list(small_cases()), ids=lambda s: f"{s.name}-{s.mx}x{s.my}-p{s.p}")
+def test_fp64_bit_identical_to_oracle(oracle, sc):
+    b, times, rows, u, v, a = gpu_run(sc)
+    to, ro, uo, vo, ao = oracle_run(oracle, sc, b)
+    np.testing.assert_array_equal(times, to)
+    np.testing.assert_array_equal(rows, ro)
+    np.testing.assert_array_equal(u, uo)
+    np.testing.assert_array_equal(v, vo)
+    np.testing.assert_array_equal(a, ao)
+
+
+@pytest.mark.parametrize("name", ["small_p3", "small_p4_force", "small_p8"])
+def test_fp64_golden_fixtures(name):
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    sc = {"small_p3": lambda: SC.small(mx=6, my=5, p=3, nt=120),
+          "small_p4_force": lambda: SC.small(mx=7, my=3, p=4, kind=S.SOURCE_FORCE, nt=100),
+          "small_p8": lambda: SC.small(mx=4, my=3, p=8, seed=5, nt=80)}[name]()
+    b, times, rows, u, v, a = gpu_run(sc)
+    assert b.dt == float(z["dt"])
+    np.testing.assert_array_equal(rows, z["trace"])
+    np.testing.assert_array_equal(u, z["u"])
+    np.testing.assert_array_equal(v, z["v"])
+    np.testing.assert_array_equal(a, z["a"])
+
+
+def test_config1_fp64_matches_golden_and_fp32_error():
+    """64x64 elements, p=6, point force, 1000 steps at 0.2 CFL."""
+    z = np.load(os.path.join(GOLDEN, "config1.npz"))
+    b, times, rows, _, _, _ = gpu_run(SC.config1())
+    assert b.nsteps == 1000
+    np.testing.assert_array_equal(times, z["times"])
+    np.testing.assert_array_equal(rows, z["trace"])
+    for prec, tol_uv in [(S.FP32_KAHAN, 1e-5), (S.FP32, 3e-5)]:
+        _, _, r32, _, _, _ = gpu_run(SC.config1(), precision=prec)
+        assert rel_l2(r32[..., 0:2], rows[..., 0:2]) < tol_uv  # displacement
+        assert rel_l2(r32[..., 2:4], rows[..., 2:4]) < tol_uv  # velocity
+
+
+def test_readme_demo_figure():
+    """The README example (-mx 64 -my 64 -bdegree 6 -tmax 0.4, demo dt) against
+    the oracle fixture, and the features of the reference's vy figure
+    (docs/figs/se2wave-demo-vy.png): onset ~0.1 s, +5.2e-9 near 0.16 s,
+    -6e-9 near 0.187 s."""
+    z = np.load(os.path.join(GOLDEN, "demo_readme.npz"))
+    b, times, rows, _, _, _ = gpu_run(SC.demo(mx=64, p=6, nt=100000, tmax=0.4, cfl_factor=0.5), chunks=7)
+    np.testing.assert_array_equal(rows, z["trace"])
+    vy = rows[:, 0, 3]
+    i, j = np.argmax(vy), np.argmin(vy)
+    assert 5.0e-9 < vy[i] < 5.5e-9 and 0.155 < times[i] < 0.166
+    assert -6.3e-9 < vy[j] < -5.7e-9 and 0.182 < times[j] < 0.192
+
+
+def test_split_schedule_equals_fused(oracle):
+    sc = SC.small(mx=9, my=7, p=4, nt=30)
+    _, _, r1, u1, v1, a1 = gpu_run(sc, schedule=S.SCHED_FUSED)
+    _, _, r2, u2, v2, a2 = gpu_run(sc, schedule=S.SCHED_SPLIT)
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_array_equal(u1, u2)
+    np.testing.assert_array_equal(a1, a2)
+    for prec in [S.FP32, S.FP32_KAHAN]:
+        _, _, r1, u1, _, _ = gpu_run(sc, precision=prec, schedule=S.SCHED_FUSED)
+        _, _, r2, u2, _, _ = gpu_run(sc, precision=prec, schedule=S.SCHED_SPLIT)
+        assert rel_l2(u2, u1) < 1e-5
+
+
+def test_chunked_advance_equals_single_advance():
+    """Runs restart from the plain state (u, v, a); chunking must not change bits."""
+    sc = SC.small(mx=6, my=5, p=3, nt=60)
+    _, t1, r1, u1, v1, a1 = gpu_run(sc)
+    _, t2, r2, u2, v2, a2 = gpu_run(sc, chunks=7)
+    np.testing.assert_array_equal(t1, t2)
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_array_equal(u1, u2)
+    np.testing.assert_array_equal(v1, v2)
+    np.testing.assert_array_equal(a1, a2)
+
+
+def test_state_roundtrip_and_restart(oracle):
+    sc = SC.small(mx=6, my=5, p=3, nt=30)
+    rng = np.random.default_rng(3)
+    u0, v0, a0 = (rng.standard_normal(2 * sc.nnodes) * 1e-3 for _ in range(3))
+    b = SC.build(sc)
+    b.solver.ctx.set_state(u0, v0, a0)
+    u, v, a = b.solver.ctx.get_state()
+    np.testing.assert_array_equal(u, u0)
+    np.testing.assert_array_equal(v, v0)
+    np.testing.assert_array_equal(a, a0)
+    _, rows = b.solver.advance(30, b.dt)
+    pr = oracle.Problem(sc, dt=b.dt, nt=30)
+    _, ro, uo, vo, ao = pr.run(u0=u0, v0=v0, a0=a0)
+    np.testing.assert_array_equal(rows, ro)
+    np.testing.assert_array_equal(b.solver.ctx.get_state()[0], uo)
+
+
+def test_assemble_forces(oracle):
+    for sc in [SC.small(mx=9, my=7, p=4), SC.small(mx=5, my=6, p=7, seed=9)]:
+        u = np.random.default_rng(sc.p).standard_normal(2 * sc.nnodes)
+        fo = oracle.Problem(sc, dt=1e-5).assemble(u)
+        for prec in [S.FP64, S.FP32, S.FP32_KAHAN]:
+            built = SC.build(sc, precision=prec)
+            f = built.solver.ctx.assemble_forces(u)
+            if prec == S.FP64:
+                np.testing.assert_array_equal(f, fo)
+            else:
+                assert rel_l2(f, fo) < 2e-6
+
+
+def test_fp32_kernel_variants_agree():
+    """Register-blocked schedules share the arithmetic: identical bits.  The
+    LDS-staged kernel sums in another order: agreement to FP32 rounding."""
+    hip, _ = S.libraries()
+    sc = SC.small(mx=37, my=33, p=4, nt=60)
+    for prec in [S.FP32, S.FP32_KAHAN]:
+        _, _, r0, u0, _, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_LDS)
+        ref = None
+        for k in range(S.KERNEL_REG, hip.se2w_kernel_count()):
+            try:
+                _, _, r, u, v, a = gpu_run(sc, precision=prec, kernel=k)
+            except S.Se2wError:
+                continue
+            if ref is None:
+                ref = (r, u)
+            np.testing.assert_array_equal(r, ref[0])
+            np.testing.assert_array_equal(u, ref[1])
+        assert rel_l2(ref[1], u0) < 2e-6
+
+
+def test_autotune_keeps_the_state():
+    sc = SC.small(mx=40, my=36, p=4, nt=50)
+    b1 = SC.build(sc, precision=S.FP32_KAHAN)
+    b1.solver.advance(20, b1.dt)
+    best, ms = b1.solver.ctx.autotune(steps=5, reps=2)
+    assert best >= 0
+    _, r1 = b1.solver.advance(30, b1.dt)
+    u1 = b1.solver.ctx.get_state()[0]
+    b2 = SC.build(sc, precision=S.FP32_KAHAN)
+    b2.solver.ctx.set_kernel(best)
+    b2.solver.advance(20, b2.dt)
+    _, r2 = b2.solver.advance(30, b2.dt)
+    np.testing.assert_array_equal(r1, r2)
+    np.testing.assert_array_equal(u1, b2.solver.ctx.get_state()[0])
+
+
+def test_norms():
+    sc = SC.small(mx=6, my=5, p=3, nt=40)
+    b, _, _, u, v, _ = gpu_run(sc)
+    n = b.solver.ctx.norms()
+    np.testing.assert_allclose(n, [u.max(), u.min(), np.linalg.norm(u), v.max(), v.min(), np.linalg.norm(v)],
+                               rtol=1e-13)
+
+
+def test_receiver_files_byte_identical(oracle, tmp_path):
+    sc = SC.small(mx=6, my=5, p=3, nt=40)
+    b, times, rows, _, _, _ = gpu_run(sc)
+    (tmp_path / "gpu").mkdir()
+    (tmp_path / "orc").mkdir()
+    b.receivers.write_header(str(tmp_path / "gpu"))
+    b.receivers.append_rows(times[:17], rows[:17])
+    b.receivers.append_rows(times[17:], rows[17:])
+    to, ro, _, _, _ = oracle_run(oracle, sc, b)
+    oracle.Problem(sc, dt=b.dt).write_receiver_files(str(tmp_path / "orc"), to, ro)
+    for f in ["closestqpsource-receiverCP-uva-6x5-p3-rank0.dat", "closestqpsource-receiverCP-uva-6x5-p3.mpimeta"]:
+        assert (tmp_path / "gpu" / f).read_bytes() == (tmp_path / "orc" / f).read_bytes()
+
+
+def test_program_demo_run_matches_oracle(oracle, tmp_path):
+    """The se2wave executable with the reference's demo options writes the
+    same receiver file as the oracle run of the same problem."""
+    exe = os.path.join(ROOT, "se2dr_fork_amd", "bin", "se2wave")
+    args = ["-mx", "16", "-bdegree", "3", "-tmax", "0.05", "-cfl_factor", "0.5", "-of", "100", "-output_dir", str(tmp_path)]
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    assert "[se2wave] Using time step size:" in r.stdout and "[step" in r.stdout
+    sc = SC.demo(mx=16, p=3, nt=100000, tmax=0.05, cfl_factor=0.5)
+    b = SC.build(sc, with_solver=False)
+    pr = oracle.Problem(sc, dt=b.dt, nt=b.nsteps)
+    to, ro, _, _, _ = pr.run()
+    (tmp_path / "orc").mkdir()
+    pr.write_receiver_files(str(tmp_path / "orc"), to, ro)
+    name = "closestqpsource-receiverCP-uva-16x16-p3-rank0.dat"
+    assert (tmp_path / name).read_bytes() == (tmp_path / "orc" / name).read_bytes()
+    assert (tmp_path / f"step-{len(to):04d}.vts").exists()
+
+
+def test_config2_full_size(oracle):
+    """512x512 elements, p=4 (4.2M nodes): FP64 GPU bit-identical to the oracle
+    over a truncated run; FP32 modes against FP64 over the full 2000 steps."""
+    sc = SC.config2()
+    b, _, _, u, v, a = gpu_run(sc, nsteps=12)
+    pr = oracle.Problem(sc, dt=b.dt, nt=12)
+    _, _, uo, vo, ao = pr.run()
+    np.testing.assert_array_equal(u, uo)
+    np.testing.assert_array_equal(v, vo)
+    np.testing.assert_array_equal(a, ao)
+    _, _, r64, _, _, _ = gpu_run(sc)
+    _, _, r32k, _, _, _ = gpu_run(sc, precision=S.FP32_KAHAN)
+    assert rel_l2(r32k[..., 0:2], r64[..., 0:2]) < 1e-5
+    assert rel_l2(r32k[..., 2:4], r64[..., 2:4]) < 1e-5
+
+
+def test_gpu_error_paths():
+    ctx = S.Context(4, 4, 3, precision=S.FP32)
+    with pytest.raises(S.Se2wError, match="set tables, cells and mass"):
+        ctx.run(5, 1e-4)
+    with pytest.raises(S.Se2wError, match="out of range"):
+        ctx.set_sources([16], np.zeros(2 * 16))
+    with pytest.raises(S.Se2wError, match="gll"):
+        ctx.set_receivers([0], [16])
+    with pytest.raises(S.Se2wError, match="register-blocked"):
+        S.Context(4, 4, 3, precision=S.FP64).set_kernel(S.KERNEL_REG)
+    with pytest.raises(S.Se2wError, match="p = 9"):
+        S.Context(4, 4, 9)
