@@ -38,6 +38,68 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
 
 
+# receiver record columns: ux uy vx vy ax ay curl
+FIELDS = {"u": [0, 1], "v": [2, 3], "a": [4, 5], "curl": [6]}
+
+
+def field_errors(tr, ref):
+    return {k: rel_l2(tr[:, :, c], ref[:, :, c]) for k, c in FIELDS.items()}
+
+
+def stream_copy_gbs(nbytes=1 << 30, reps=20):
+    """Device-to-device copy bandwidth (read + write bytes / time) of a plain copy kernel."""
+    import torch
+
+    n = nbytes // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return 2 * nbytes / (ms * 1e-3) / 1e9
+
+
+def measured_traffic(mx, p, prec, signature):
+    """HBM bytes per launch of the step kernel, from the committed rocprofv3 counter summary."""
+    path = os.path.join(ROOT, "profiles", "step_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        rows = json.load(f).get("rows", [])
+    for r in rows:
+        if r.get("mx") == mx and r.get("p") == p and r.get("precision") == prec and signature in r.get("kernel", ""):
+            return r.get("traffic_bytes")
+    return None
+
+
+# (threads in x, threads in y, minimum waves per SIMD) of the register-blocked
+# kernel's schedules, in the library's variant order (SE2W_KERNEL_REG + k)
+REG_SCHEDULES = [(32, 4, 0), (16, 4, 0), (32, 2, 0), (16, 8, 0), (64, 2, 0), (16, 4, 3), (32, 4, 3), (16, 16, 0)]
+
+
+def kernel_signature(variant, p, mode):
+    """The step kernel's template signature as rocprofv3 prints it."""
+    if variant == 0:
+        return f"tile_kernel<{p}, {mode}, true>"
+    ex, ey, w = REG_SCHEDULES[variant - 1]
+    return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}>"
+
+
+def kernel_name(variant, p, mode):
+    what = ("LDS-staged tile" if variant == 0 else
+            "thread per element, %dx%d element threads" % REG_SCHEDULES[variant - 1][:2])
+    return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
+
+
 def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, kernel=None):
     b = SC.build(sc, precision=prec)
     sv = b.solver
@@ -54,8 +116,9 @@ def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, ker
     sv.advance(profile_steps, b.dt)
     _, kern = sv.ctx.timing()
     sv.ctx.set_profiling(False)
+    variant = sv.ctx.kernel()
     sv.close()
-    out = {"wall_s": wall, "event_ms": run_ms, "kernel_ms": float(kern[0]), "dt": b.dt}
+    out = {"wall_s": wall, "event_ms": run_ms, "kernel_ms": float(kern[0]), "dt": b.dt, "variant": variant}
     if full_steps > 0:
         bf = SC.build(sc, precision=prec)
         if kernel is not None:
@@ -67,21 +130,28 @@ def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, ker
 
 
 def cpu_baseline(sc, dt, seconds_target=15.0):
-    """The reference algorithm (dense FP64 element loop, oracle/) on the host."""
+    """The reference algorithm (dense FP64 element loop, oracle/) on the host.
+
+    Setup (mesh, tables, mass) is excluded: a 1-step run and an n-step run
+    are timed and the rate is taken over their difference of n - 1 steps."""
     from oracle import oracle as O
     cores = min(os.cpu_count() or 1, 16)
-    pr = O.Problem(sc, dt=dt, nthreads=cores, nt=1)
-    t0 = time.perf_counter()
-    pr.run(max_steps=1, want_state=False)
-    one = time.perf_counter() - t0
-    nsteps = max(1, min(200, int(seconds_target / max(one, 1e-3))))
-    pr = O.Problem(sc, dt=dt, nthreads=cores, nt=nsteps)
-    t0 = time.perf_counter()
-    pr.run(max_steps=nsteps, want_state=False)
-    el = time.perf_counter() - t0
-    return {"value": sc.nnodes * nsteps / el, "unit": "node-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{nsteps} steps of the 512x512 p=4 workload (FP64 dense reference element loop, "
-                      f"oracle/ref_cpu.c, {cores} OpenMP threads), {el:.1f} s"}
+
+    def timed(n):
+        pr = O.Problem(sc, dt=dt, nthreads=cores, nt=n)
+        t0 = time.perf_counter()
+        pr.run(max_steps=n, want_state=False)
+        return time.perf_counter() - t0
+
+    t1 = timed(1)
+    t3 = timed(3)
+    per_step = max((t3 - t1) / 2, 1e-4)
+    n = int(max(4, min(1000, seconds_target / per_step)))
+    tn = timed(n)
+    el = max(tn - t1, 1e-9)
+    return {"value": sc.nnodes * (n - 1) / el, "unit": "node-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{n - 1} steps of the {sc.mx}x{sc.my} p={sc.p} workload (FP64 dense reference element loop, "
+                      f"oracle/ref_cpu.c, {cores} OpenMP threads): {n}-step run minus 1-step run = {el:.1f} s"}
 
 
 def main():
@@ -94,6 +164,7 @@ def main():
     ap.add_argument("--mx", type=int, default=512)
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-copy", action="store_true", help="skip the device copy-bandwidth measurement")
     ap.add_argument("--precisions", default="fp32k,fp32,fp64")
     ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
                     help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
@@ -123,7 +194,7 @@ def main():
         ref = res[S.FP64]["trace"]
         for p in precs:
             if p != S.FP64:
-                errs[p] = rel_l2(res[p]["trace"], ref)
+                errs[p] = field_errors(res[p]["trace"], ref)
     names = {0: "fp64", 1: "fp32", 2: "fp32k"}
     out = {
         "metric": METRIC,
@@ -143,15 +214,20 @@ def main():
                                f"16 receivers, {names[head]} arithmetic, fused step kernel",
                    "mx": args.mx, "my": args.mx, "p": args.p, "nodes": nodes},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None, "traffic": None,
-                     "kernel": "tile_kernel (fused element operator + Verlet update)",
+                     "frac": (achieved / HBM_PEAK_GBS) if achieved else None,
+                     "traffic": measured_traffic(args.mx, args.p, names[head],
+                                                 kernel_signature(res[head]["variant"], args.p, head)),
+                     "kernel": kernel_name(res[head]["variant"], args.p, head),
                      "bytes_per_node": BYTES_PER_NODE[head], "kernel_ms": kern_ms},
         "node_updates_per_s": {names[p]: r for p, r in rates.items()},
         "canonical_roofline_frac": {names[p]: rates[p] * CANONICAL_B_NODE[p] / (HBM_PEAK_GBS * 1e9) for p in rates},
         "kernel_ms": {names[p]: r["kernel_ms"] for p, r in res.items()},
         "receiver_rel_l2_vs_fp64": {names[p]: e for p, e in errs.items()},
-        "full_steps": args.full_steps,
+        "receiver_error_steps": args.full_steps,
+        "algorithmic_bytes_per_launch": BYTES_PER_NODE[head] * nodes,
     }
+    if not args.no_copy:
+        out["stream_copy_GBs"] = stream_copy_gbs()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"])
     else:

@@ -109,9 +109,11 @@ int se2w_assemble_forces(se2w_ctx *ctx, const double *u, double *f);
 int se2w_norms(se2w_ctx *ctx, double out[6]);
 
 /* Wall time of the step loop of the last se2w_run (HIP events on the
- * context's stream, predictor included) and, when per-launch profiling is
- * on, the mean duration of the step kernel(s).  kernel_ms[0] = fused or
- * element kernel, kernel_ms[1] = update kernel (split schedule). */
+ * context's stream, predictor included) and, when profiling is on, the mean
+ * duration of the step kernel(s): kernel_ms[0] = fused step kernel (events
+ * around the whole sequence of step launches, divided by the step count) or
+ * element kernel, kernel_ms[1] = update kernel (split schedule; events
+ * around each launch). */
 int se2w_set_profiling(se2w_ctx *ctx, int32_t on);
 int se2w_get_timing(se2w_ctx *ctx, double *run_ms, double kernel_ms[2]);
 

@@ -83,7 +83,7 @@ template <typename R> struct StepParams {
   const R *psi;       // [nsteps_run][ns]
   int ns;
   // receivers (CSR per tile)
-  const int *tile_rec_ptr, *tile_rec_idx, *rec_elem;
+  const int *tile_rec_ptr, *tile_rec_idx, *rec_elem, *rec_gll;
   R *rec;             // [nsteps_run][nr][npe][6]
   int nr;
   int step;           // step index inside the current run
@@ -229,7 +229,14 @@ __device__ __forceinline__ void force_f32(const float *Sxx, const float *Syy, co
   fy = -(iJ00 * ry + iJ11 * cy);
 }
 
-// Receiver sampling: every node of a receiver's element records (u, v, a).
+// Receiver sampling.  Per receiver and step the record holds what the
+// reference's RecordUVA row needs (se2wave.c:2922-2959): u, v, a at the
+// receiver's GLL node (gi, gj) of its element, vx on the element's column gi
+// and vy on its row gj (for curl v = dvy/dx - dvx/dy, formed on the host).
+// Layout per (step, receiver): [ux uy vx vy ax ay | vx(gi, j), j = 0..p |
+// vy(i, gj), i = 0..p], i.e. rec_stride(P) = 6 + 2 (P + 1) values.
+__host__ __device__ constexpr int rec_stride(int p) { return 6 + 2 * (p + 1); }
+
 template <int P, typename R>
 __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int tile, int gx, int gy, R ux, R uy,
                                                  R vx, R vy, R ax, R ay) {
@@ -237,10 +244,13 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int t
   const int r0 = prm.tile_rec_ptr[tile], r1 = prm.tile_rec_ptr[tile + 1];
   for (int t = r0; t < r1; t++) {
     const int r = prm.tile_rec_idx[t];
-    const int e = prm.rec_elem[r];
+    const int e = prm.rec_elem[r], g = prm.rec_gll[r];
     const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
-    if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
-      R *o = prm.rec + (((long)prm.step * prm.nr + r) * (NB * NB) + li + lj * NB) * 6;
+    const int gi = g % NB, gj = g / NB;
+    R *o = prm.rec + ((long)prm.step * prm.nr + r) * rec_stride(P);
+    if (li == gi && lj >= 0 && lj <= P) o[6 + lj] = vx;
+    if (lj == gj && li >= 0 && li <= P) o[6 + NB + li] = vy;
+    if (li == gi && lj == gj) {
       o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy; o[4] = ax; o[5] = ay;
     }
   }

@@ -313,3 +313,26 @@ def test_energy_is_conserved_after_the_source(oracle):
         energies.append(0.5 * v @ (M * v) - 0.5 * u @ f - dt * dt / 8.0 * (f @ (f / M)))
     e = np.array(energies)
     assert np.ptp(e) < 1e-10 * e.mean()
+
+
+def test_oracle_reproduces_the_reference_figure():
+    """The reference's only result artefact is the README plot of vy at the
+    first station of the demo (docs/figs/se2wave-demo-vy.png; run
+    -mx 64 -my 64 -bdegree 6 -tmax 0.4, README.md:40-52): first arrival near
+    0.1 s, a positive peak of about +5.2e-9 near 0.16 s, the largest trough of
+    about -6e-9 near 0.19 s, then a smaller second peak and trough before
+    0.35 s.  The oracle's run of that case is the committed fixture."""
+    import os
+
+    from conftest import GOLDEN
+
+    z = np.load(os.path.join(GOLDEN, "demo_readme.npz"))
+    t, vy = z["times"], z["trace"][:, 0, 3]
+    i, j = np.argmax(vy), np.argmin(vy)
+    assert 5.0e-9 < vy[i] < 5.5e-9 and 0.155 < t[i] < 0.166
+    assert -6.3e-9 < vy[j] < -5.7e-9 and 0.182 < t[j] < 0.192
+    onset = t[np.argmax(np.abs(vy) > 0.02 * np.abs(vy).max())]
+    assert 0.09 < onset < 0.12
+    late = (t > 0.2) & (t < 0.35)
+    assert 2.5e-9 < vy[late].max() < 3.5e-9 and -3.3e-9 < vy[late].min() < -2.4e-9
+    assert t[-1] == pytest.approx(0.4, abs=z["dt"])
