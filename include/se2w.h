@@ -141,6 +141,40 @@ int se2w_autotune(se2w_ctx *ctx, int32_t steps, int32_t reps, int32_t *best_vari
 /* Tile shape (owned elements per workgroup) used by the step kernels. */
 int se2w_get_tile(se2w_ctx *ctx, int32_t *tex, int32_t *tey);
 
+/* ------------------------------------------------------------------------
+ * Ensembles (no counterpart in the reference, which runs one instance per
+ * program): contexts of one device, degree and precision -- each with its own
+ * mesh size, materials, sources, receivers and dt -- advanced together, one
+ * batched kernel launch per step, the launch sequence of a run captured in a
+ * HIP graph and replayed.  Members keep their own state and receiver record:
+ * after a run, se2w_get_receivers / se2w_get_state on each member.  The
+ * ensemble does not own its members; destroy it before them.
+ * ------------------------------------------------------------------------ */
+typedef struct se2w_ensemble se2w_ensemble;
+
+/* members[0..n-1]: fused-schedule contexts with tables, cells and mass set. */
+int se2w_ensemble_create(int32_t n, se2w_ctx *const *members, se2w_ensemble **ens);
+/* nsteps steps of every member: dt[i] its time step, psi[i] its
+ * source-time table [nsteps][ns_i] (may be NULL for members without sources). */
+int se2w_ensemble_run(se2w_ensemble *ens, int32_t nsteps, const double *dt, const double *const *psi);
+/* Kernel variant of the batched step (SE2W_KERNEL_LDS or SE2W_KERNEL_REG + k),
+ * applied to every member. */
+int se2w_ensemble_set_kernel(se2w_ensemble *ens, int32_t variant);
+int se2w_ensemble_get_kernel(se2w_ensemble *ens, int32_t *variant);
+/* Times every built variant over `steps` batched steps (reps repetitions,
+ * state restored afterwards) and keeps the fastest. */
+int se2w_ensemble_autotune(se2w_ensemble *ens, int32_t steps, int32_t reps, int32_t *best_variant, double *best_ms);
+/* on = 1 (default): replay the run's launches from a captured HIP graph;
+ * on = 0: launch them one by one. */
+int se2w_ensemble_set_graph(se2w_ensemble *ens, int32_t on);
+/* Last run: whole run (predictors included) and mean time per batched step. */
+int se2w_ensemble_get_timing(se2w_ensemble *ens, double *run_ms, double *step_ms);
+/* Tiles of all members and workgroups of the persistent batched launch. */
+int se2w_ensemble_get_launch(se2w_ensemble *ens, int32_t *tiles, int32_t *grid);
+/* Member i of the ensemble. */
+int se2w_ensemble_get_member(se2w_ensemble *ens, int32_t i, se2w_ctx **member);
+int se2w_ensemble_destroy(se2w_ensemble *ens);
+
 int se2w_destroy(se2w_ctx *ctx);
 const char *se2w_strerror(int code);
 const char *se2w_last_error(void);

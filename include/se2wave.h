@@ -177,6 +177,11 @@ double se2w_solver_time(SE2WSolver solver);
 int se2w_solver_steps_done(SE2WSolver solver);
 /* times[nsteps] and trace[nsteps][nr][7] of the last advance. */
 int se2w_solver_get_trace(SE2WSolver solver, double *times, double *trace);
+/* Advances n solvers together, one batched GPU launch per step for all of
+ * them; dt[i] is solver i's step.  ens is an ensemble of the solvers'
+ * contexts (se2w_solver_context) in the same order.  Each solver's time, step
+ * count and trace advance as with se2w_solver_advance. */
+int se2w_solvers_advance(int n, SE2WSolver *solvers, se2w_ensemble *ens, int nsteps, const double *dt);
 
 /* Runs the se2wave program (main + se2wave_demo) with reference options. */
 int se2w_main(int argc, char **argv);

@@ -83,6 +83,16 @@ def _declare(hip, host):
     hip.se2w_get_kernel.argtypes = [v, _ip]
     hip.se2w_autotune.argtypes = [v, i32, i32, _ip, _dp]
     hip.se2w_destroy.argtypes = [v]
+    hip.se2w_ensemble_create.argtypes = [i32, C.POINTER(v), C.POINTER(v)]
+    hip.se2w_ensemble_run.argtypes = [v, i32, _dp, C.POINTER(_dp)]
+    hip.se2w_ensemble_set_kernel.argtypes = [v, i32]
+    hip.se2w_ensemble_get_kernel.argtypes = [v, _ip]
+    hip.se2w_ensemble_autotune.argtypes = [v, i32, i32, _ip, _dp]
+    hip.se2w_ensemble_set_graph.argtypes = [v, i32]
+    hip.se2w_ensemble_get_timing.argtypes = [v, _dp, _dp]
+    hip.se2w_ensemble_get_launch.argtypes = [v, _ip, _ip]
+    hip.se2w_ensemble_get_member.argtypes = [v, i32, C.POINTER(v)]
+    hip.se2w_ensemble_destroy.argtypes = [v]
     host.se2w_host_last_error.restype = C.c_char_p
     host.se2w_gll.argtypes = [C.c_int, _dp, _dp]
     host.se2w_basis1d.argtypes = [C.c_int, C.c_int, _dp, _dp]
@@ -129,6 +139,7 @@ def _declare(hip, host):
     host.se2w_solver_time.argtypes = [v]
     host.se2w_solver_time.restype = d
     host.se2w_solver_get_trace.argtypes = [v, _dp, _dp]
+    host.se2w_solvers_advance.argtypes = [C.c_int, C.POINTER(v), v, C.c_int, _dp]
     host.se2w_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
 
 
@@ -502,6 +513,73 @@ class Context:
         a, b = C.c_int32(), C.c_int32()
         _gchk(self._hip.se2w_get_tile(self._handle, C.byref(a), C.byref(b)))
         return a.value, b.value
+
+
+class Ensemble:
+    """Solvers of one degree and precision advanced together: one batched GPU
+    launch per step for all members (each with its own mesh, materials,
+    sources, receivers and dt), the run's launches replayed from a HIP graph."""
+
+    def __init__(self, solvers: Sequence[Solver]):
+        self._hip, self._host = libraries()
+        self.solvers = list(solvers)
+        n = len(self.solvers)
+        ctxs = (C.c_void_p * n)(*[sv.ctx._handle for sv in self.solvers])
+        self.h = C.c_void_p()
+        _gchk(self._hip.se2w_ensemble_create(n, ctxs, C.byref(self.h)))
+        self._sv = (C.c_void_p * n)(*[sv.h for sv in self.solvers])
+
+    def advance(self, nsteps: int, dt):
+        """nsteps steps of every member (dt: one step size per member); returns
+        per member (times, trace[nsteps][nr][7]) as Solver.advance does."""
+        dts = np.ascontiguousarray(dt, dtype=np.float64)
+        _hchk(self._host.se2w_solvers_advance(len(self.solvers), self._sv, self.h, nsteps, _ptr(dts)))
+        out = []
+        for sv in self.solvers:
+            times = np.zeros(nsteps)
+            trace = np.zeros(nsteps * max(sv.nr, 1) * 7)
+            _hchk(self._host.se2w_solver_get_trace(sv.h, _ptr(times), _ptr(trace)))
+            out.append((times, trace[: nsteps * sv.nr * 7].reshape(nsteps, sv.nr, 7)))
+        return out
+
+    def set_kernel(self, variant: int):
+        _gchk(self._hip.se2w_ensemble_set_kernel(self.h, variant))
+
+    def kernel(self) -> int:
+        v = C.c_int32()
+        _gchk(self._hip.se2w_ensemble_get_kernel(self.h, C.byref(v)))
+        return v.value
+
+    def autotune(self, steps=10, reps=3):
+        best, ms = C.c_int32(), C.c_double()
+        _gchk(self._hip.se2w_ensemble_autotune(self.h, steps, reps, C.byref(best), C.byref(ms)))
+        return best.value, ms.value
+
+    def set_graph(self, on: bool):
+        _gchk(self._hip.se2w_ensemble_set_graph(self.h, 1 if on else 0))
+
+    def timing(self):
+        """(whole last run in ms, mean batched step in ms)."""
+        a, b = C.c_double(), C.c_double()
+        _gchk(self._hip.se2w_ensemble_get_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def launch(self):
+        """(tiles of all members, workgroups of the persistent launch)."""
+        a, b = C.c_int32(), C.c_int32()
+        _gchk(self._hip.se2w_ensemble_get_launch(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self._hip.se2w_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run_program(args: Sequence[str]) -> int:

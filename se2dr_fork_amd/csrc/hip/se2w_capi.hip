@@ -15,6 +15,7 @@
 #include "se2w.h"
 #include "se2w_kernels.hpp"
 #include "se2w_kernels_reg.hpp"
+#include "se2w_ens.hpp"
 
 using namespace se2w;
 
@@ -286,10 +287,6 @@ StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double d
   return p;
 }
 
-// Schedules instantiated for (p, fused): all of them for the degree of the
-// headline workload, the first three elsewhere, the first one for the split
-// schedule's element kernel.
-constexpr bool reg_schedule_built(int P, int S, bool fused) { return fused ? (P == 4 || S < 3) : S == 0; }
 
 bool reg_variant_built(int p, int variant, int schedule) {
   return is_reg_variant(variant) && reg_schedule_built(p, reg_schedule_of(variant), schedule == SE2W_SCHED_FUSED);
@@ -705,7 +702,13 @@ int se2w_get_state(se2w_ctx *c, double *u, double *v, double *a) {
   return SE2W_OK;
 }
 
-int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
+}  // extern "C"
+
+namespace {
+
+// Checks a run's arguments and stages its inputs: the source-time table and
+// the receiver record buffer.
+int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   if (nsteps < 1) return fail(SE2W_ERR_ARG, "nsteps must be >= 1");
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SE2W_ERR_ARG, "dt must be positive and finite");
@@ -744,6 +747,16 @@ int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
   }
   c->rec_steps = nsteps;
   c->trace.clear();
+  return SE2W_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
+  int ierr = prepare_run(c, nsteps, dt, psi);
+  if (ierr) return ierr;
   if (c->mode == SE2W_FP64) ierr = run_mode<MODE_F64>(c, nsteps, dt);
   else if (c->mode == SE2W_FP32) ierr = run_mode<MODE_F32>(c, nsteps, dt);
   else ierr = run_mode<MODE_F32K>(c, nsteps, dt);
@@ -903,7 +916,50 @@ int size_bucket(int n) {
   while (n > 1) { n >>= 1; b++; }
   return b;
 }
+// Copy of a context's state (u, v, a and the compensation of the current
+// buffer) kept on the device while trial runs overwrite it.
+struct Snapshot {
+  void *src[4] = {nullptr, nullptr, nullptr, nullptr};
+  void *copy[4] = {nullptr, nullptr, nullptr, nullptr};
+  size_t bytes = 0;
+  int cur = 0, ns = 0, nr = 0;
+};
+
+void snapshot_free(Snapshot &sn) {
+  for (void *&x : sn.copy) dev_free(x);
+}
+
+int snapshot_take(se2w_ctx *c, Snapshot &sn) {
+  sn.bytes = (size_t)c->pitch * c->ny * 2 * real_size(c->mode);
+  void *srcs[4] = {c->U[c->cur], c->V, c->A, c->C[c->cur]};
+  for (int f = 0; f < 4; f++) {
+    sn.src[f] = srcs[f];
+    if (!srcs[f]) continue;
+    int ierr = dev_alloc(&sn.copy[f], sn.bytes);
+    if (ierr) { snapshot_free(sn); return ierr; }
+    HIPCHK(hipMemcpyAsync(sn.copy[f], srcs[f], sn.bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  HIPCHK(hipStreamSynchronize(c->stream));
+  sn.cur = c->cur;
+  sn.ns = c->ns;
+  sn.nr = c->nr;
+  return SE2W_OK;
+}
+
+int snapshot_restore(se2w_ctx *c, Snapshot &sn) {
+  c->cur = sn.cur;
+  for (int f = 0; f < 4; f++)
+    if (sn.copy[f]) HIPCHK(hipMemcpyAsync(sn.src[f], sn.copy[f], sn.bytes, hipMemcpyDeviceToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  snapshot_free(sn);
+  c->ns = sn.ns;
+  c->nr = sn.nr;
+  c->rec_steps = 0;
+  return SE2W_OK;
+}
+
 }  // namespace
+
 
 int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_variant, double *best_ms) {
   if (!c || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
@@ -925,20 +981,9 @@ int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_varian
   if (reg_available(c->p, c->mode))
     for (int k = 0; k < kNumRegSchedules; k++)
       if (reg_variant_built(c->p, SE2W_KERNEL_REG + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG + k);
-  // snapshot the state
-  const size_t field = (size_t)c->pitch * c->ny * 2 * real_size(c->mode);
-  std::vector<void *> snap;
-  void *srcs[4] = {c->U[c->cur], c->V, c->A, c->C[c->cur]};
-  for (void *src : srcs) {
-    void *d = nullptr;
-    if (src) {
-      int ierr = dev_alloc(&d, field);
-      if (ierr) { for (void *x : snap) hipFree(x); return ierr; }
-      HIPCHK(hipMemcpyAsync(d, src, field, hipMemcpyDeviceToDevice, c->stream));
-    }
-    snap.push_back(d);
-  }
-  const int cur0 = c->cur, variant0 = c->variant, ns0 = c->ns, nr0 = c->nr;
+  Snapshot sn;
+  if (int ierr = snapshot_take(c, sn)) return ierr;
+  const int variant0 = c->variant;
   std::vector<double> zeros((size_t)steps * std::max(c->ns, 1), 0.0);
   int best = variant0;
   double best_t = 1e300;
@@ -958,15 +1003,7 @@ int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_varian
     if (med < best_t) { best_t = med; best = v; }
   }
   c->profiling = prof0;
-  // restore the state and the receiver bookkeeping
-  c->cur = cur0;
-  for (int f = 0; f < 4; f++)
-    if (snap[f]) HIPCHK(hipMemcpyAsync(srcs[f], snap[f], field, hipMemcpyDeviceToDevice, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
-  for (void *x : snap) if (x) hipFree(x);
-  c->ns = ns0;
-  c->nr = nr0;
-  c->rec_steps = 0;
+  if (int ierr = snapshot_restore(c, sn)) return ierr;
   int ierr = se2w_set_kernel(c, best);
   if (ierr) return ierr;
   {
@@ -988,6 +1025,310 @@ int se2w_get_tile(se2w_ctx *c, int32_t *tex, int32_t *tey) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   if (tex) *tex = c->tex;
   if (tey) *tey = c->tey;
+  return SE2W_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Ensembles: many contexts of one device, degree and precision advanced
+// together, one batched launch per step.
+// ---------------------------------------------------------------------------
+struct se2w_ensemble {
+  std::vector<se2w_ctx *> members;
+  int device = 0, p = 0, mode = 0, variant = SE2W_KERNEL_LDS;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
+  void *d_prms = nullptr;
+  int *d_tile_off = nullptr;
+  int total = 0, grid = 0, cus = 0;
+  bool use_graph = true;
+  hipGraphExec_t exec = nullptr;
+  int exec_steps = 0;
+  double run_ms = 0.0, step_ms = 0.0;
+};
+
+namespace {
+
+void ens_drop_graph(se2w_ensemble *e) {
+  if (e->exec) hipGraphExecDestroy(e->exec);
+  e->exec = nullptr;
+  e->exec_steps = 0;
+}
+
+bool ens_variant_built(const se2w_ensemble *e, int variant) {
+  int bpc = 0, thr = 0;
+  return ens_occupancy(e->p, e->mode, variant, &bpc, &thr) == 0 && bpc > 0;
+}
+
+// Sets the members' tiling for `variant`, the global tile offsets and the
+// persistent grid (as many workgroups as fit the device at once, at most one
+// per tile, a multiple of 8).
+int ens_set_variant(se2w_ensemble *e, int variant) {
+  if (!ens_variant_built(e, variant))
+    return fail(SE2W_ERR_UNSUPPORTED, "kernel variant %d is not built for ensembles at p = %d", variant, e->p);
+  for (se2w_ctx *c : e->members)
+    if (int ierr = se2w_set_kernel(c, variant)) return ierr;
+  std::vector<int> off(e->members.size() + 1, 0);
+  for (size_t i = 0; i < e->members.size(); i++) off[i + 1] = off[i] + e->members[i]->ntiles;
+  e->total = off.back();
+  if (int ierr = upload<int>((void **)&e->d_tile_off, off.data(), off.size(), e->stream)) return ierr;
+  int bpc = 0, thr = 0;
+  ens_occupancy(e->p, e->mode, variant, &bpc, &thr);
+  const long fit = (long)bpc * e->cus;
+  const long want = std::min<long>(fit, ((long)e->total + 7) / 8 * 8);
+  e->grid = (int)std::max<long>(8, want / 8 * 8);
+  e->variant = variant;
+  ens_drop_graph(e);
+  return SE2W_OK;
+}
+
+template <int MODE>
+int ens_step_params(se2w_ensemble *e, const double *dt) {
+  using R = typename RealOf<MODE>::T;
+  const size_t n = e->members.size();
+  std::vector<StepParams<R>> prm(2 * n);
+  for (size_t i = 0; i < n; i++) {
+    se2w_ctx *c = e->members[i];
+    prm[2 * i] = make_params<MODE>(c, 0, dt[i], 0);
+    c->cur ^= 1;
+    prm[2 * i + 1] = make_params<MODE>(c, 0, dt[i], 0);
+    c->cur ^= 1;
+  }
+  HIPCHK(hipMemcpyAsync(e->d_prms, prm.data(), prm.size() * sizeof(StepParams<R>), hipMemcpyHostToDevice, e->stream));
+  HIPCHK(hipStreamSynchronize(e->stream));
+  return SE2W_OK;
+}
+
+template <int MODE>
+int ens_launch_steps(se2w_ensemble *e, int nsteps) {
+  EnsArgs<typename RealOf<MODE>::T> a;
+  a.prms = (const StepParams<typename RealOf<MODE>::T> *)e->d_prms;
+  a.tile_off = e->d_tile_off;
+  a.n = (int)e->members.size();
+  a.total = e->total;
+  for (int k = 0; k < nsteps; k++) {
+    a.step = k;
+    a.last = k == nsteps - 1;
+    if (ens_launch(e->p, e->mode, e->variant, &a, e->grid, e->stream))
+      return fail(SE2W_ERR_UNSUPPORTED, "no batched kernel for variant %d", e->variant);
+  }
+  HIPCHK(hipGetLastError());
+  return SE2W_OK;
+}
+
+template <int MODE>
+int ens_run_mode(se2w_ensemble *e, int nsteps, const double *dt) {
+  using R = typename RealOf<MODE>::T;
+  HIPCHK(hipEventRecord(e->ev0, e->stream));
+  // predictor of every member, then the step parameters of both parities
+  for (size_t i = 0; i < e->members.size(); i++) {
+    se2w_ctx *c = e->members[i];
+    StepParams<R> prm = make_params<MODE>(c, 0, dt[i], 0);
+    const long n = (long)c->pitch * c->ny;
+    hipLaunchKernelGGL((predictor_kernel<MODE>), dim3(grid_for(n)), dim3(kThreads), 0, e->stream, prm);
+    c->cur ^= 1;
+  }
+  HIPCHK(hipGetLastError());
+  if (int ierr = ens_step_params<MODE>(e, dt)) return ierr;
+  HIPCHK(hipEventRecord(e->ev1, e->stream));
+  if (e->use_graph) {
+    if (!e->exec || e->exec_steps != nsteps) {
+      ens_drop_graph(e);
+      hipGraph_t g = nullptr;
+      HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+      int ierr = ens_launch_steps<MODE>(e, nsteps);
+      hipError_t ce = hipStreamEndCapture(e->stream, &g);
+      if (ierr) { if (g) hipGraphDestroy(g); return ierr; }
+      if (ce != hipSuccess) return fail(SE2W_ERR_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ce));
+      hipError_t ie = hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0);
+      hipGraphDestroy(g);
+      if (ie != hipSuccess) { e->exec = nullptr; return fail(SE2W_ERR_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ie)); }
+      e->exec_steps = nsteps;
+    }
+    HIPCHK(hipGraphLaunch(e->exec, e->stream));
+  } else {
+    if (int ierr = ens_launch_steps<MODE>(e, nsteps)) return ierr;
+  }
+  HIPCHK(hipEventRecord(e->ev2, e->stream));
+  for (se2w_ctx *c : e->members) {
+    c->cur ^= (nsteps & 1);
+    if (int ierr = ensure_stage(c)) return ierr;
+  }
+  HIPCHK(hipEventSynchronize(e->ev2));
+  float a = 0.f, b = 0.f;
+  HIPCHK(hipEventElapsedTime(&a, e->ev0, e->ev2));
+  HIPCHK(hipEventElapsedTime(&b, e->ev1, e->ev2));
+  e->run_ms = a;
+  e->step_ms = b / nsteps;
+  return SE2W_OK;
+}
+
+int ens_run(se2w_ensemble *e, int nsteps, const double *dt, const double *const *psi) {
+  HIPCHK(hipSetDevice(e->device));
+  for (size_t i = 0; i < e->members.size(); i++) {
+    se2w_ctx *c = e->members[i];
+    if (int ierr = prepare_run(c, nsteps, dt[i], psi ? psi[i] : nullptr)) return ierr;
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  if (e->mode == SE2W_FP64) return ens_run_mode<MODE_F64>(e, nsteps, dt);
+  if (e->mode == SE2W_FP32) return ens_run_mode<MODE_F32>(e, nsteps, dt);
+  return ens_run_mode<MODE_F32K>(e, nsteps, dt);
+}
+
+}  // namespace
+
+extern "C" {
+
+int se2w_ensemble_create(int32_t n, se2w_ctx *const *members, se2w_ensemble **out) {
+  if (!out || n < 1 || !members) return fail(SE2W_ERR_ARG, "bad ensemble arguments");
+  *out = nullptr;
+  se2w_ctx *c0 = members[0];
+  for (int i = 0; i < n; i++) {
+    se2w_ctx *c = members[i];
+    if (!c) return fail(SE2W_ERR_ARG, "member %d is null", i);
+    for (int j = 0; j < i; j++)
+      if (members[j] == c) return fail(SE2W_ERR_ARG, "member %d appears twice", i);
+    if (c->device != c0->device || c->p != c0->p || c->mode != c0->mode)
+      return fail(SE2W_ERR_ARG, "member %d: device, degree and precision must match member 0", i);
+    if (c->schedule != SE2W_SCHED_FUSED) return fail(SE2W_ERR_UNSUPPORTED, "member %d: ensembles use the fused schedule", i);
+    if (!c->have_tables || !c->have_cells || !c->have_mass)
+      return fail(SE2W_ERR_STATE, "member %d: set tables, cells and mass first", i);
+  }
+  HIPCHK(hipSetDevice(c0->device));
+  se2w_ensemble *e = new se2w_ensemble;
+  e->members.assign(members, members + n);
+  e->device = c0->device;
+  e->p = c0->p;
+  e->mode = c0->mode;
+  int ierr = SE2W_OK;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
+      hipEventCreate(&e->ev2) != hipSuccess) {
+    se2w_ensemble_destroy(e);
+    return fail(SE2W_ERR_HIP, "stream/event creation failed");
+  }
+  e->cus = prop.multiProcessorCount;
+  const size_t pbytes = 2 * (size_t)n * (e->mode == SE2W_FP64 ? sizeof(StepParams<double>) : sizeof(StepParams<float>));
+  if ((ierr = dev_alloc(&e->d_prms, pbytes))) { se2w_ensemble_destroy(e); return ierr; }
+  const int v0 = ens_variant_built(e, c0->variant) ? c0->variant : SE2W_KERNEL_LDS;
+  if ((ierr = ens_set_variant(e, v0))) { se2w_ensemble_destroy(e); return ierr; }
+  *out = e;
+  return SE2W_OK;
+}
+
+int se2w_ensemble_run(se2w_ensemble *e, int32_t nsteps, const double *dt, const double *const *psi) {
+  if (!e || !dt) return fail(SE2W_ERR_ARG, "null argument");
+  if (nsteps < 1) return fail(SE2W_ERR_ARG, "nsteps must be >= 1");
+  return ens_run(e, nsteps, dt, psi);
+}
+
+int se2w_ensemble_set_kernel(se2w_ensemble *e, int32_t variant) {
+  if (!e) return fail(SE2W_ERR_ARG, "null ensemble");
+  HIPCHK(hipSetDevice(e->device));
+  return ens_set_variant(e, variant);
+}
+
+int se2w_ensemble_get_kernel(se2w_ensemble *e, int32_t *variant) {
+  if (!e || !variant) return fail(SE2W_ERR_ARG, "null argument");
+  *variant = e->variant;
+  return SE2W_OK;
+}
+
+int se2w_ensemble_set_graph(se2w_ensemble *e, int32_t on) {
+  if (!e) return fail(SE2W_ERR_ARG, "null ensemble");
+  e->use_graph = on != 0;
+  if (!e->use_graph) ens_drop_graph(e);
+  return SE2W_OK;
+}
+
+int se2w_ensemble_get_timing(se2w_ensemble *e, double *run_ms, double *step_ms) {
+  if (!e) return fail(SE2W_ERR_ARG, "null ensemble");
+  if (run_ms) *run_ms = e->run_ms;
+  if (step_ms) *step_ms = e->step_ms;
+  return SE2W_OK;
+}
+
+int se2w_ensemble_get_launch(se2w_ensemble *e, int32_t *tiles, int32_t *grid) {
+  if (!e) return fail(SE2W_ERR_ARG, "null ensemble");
+  if (tiles) *tiles = e->total;
+  if (grid) *grid = e->grid;
+  return SE2W_OK;
+}
+
+int se2w_ensemble_autotune(se2w_ensemble *e, int32_t steps, int32_t reps, int32_t *best_variant, double *best_ms) {
+  if (!e || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
+  HIPCHK(hipSetDevice(e->device));
+  std::vector<int> cands;
+  for (int v = 0; v < SE2W_KERNEL_REG + kNumRegSchedules; v++)
+    if (ens_variant_built(e, v)) cands.push_back(v);
+  const size_t n = e->members.size();
+  std::vector<Snapshot> sn(n);
+  for (size_t i = 0; i < n; i++) {
+    if (int ierr = snapshot_take(e->members[i], sn[i])) {
+      for (size_t j = 0; j < i; j++) snapshot_free(sn[j]);
+      return ierr;
+    }
+  }
+  std::vector<double> dts(n, 1.0e-30);
+  std::vector<std::vector<double>> zeros(n);
+  std::vector<const double *> psi(n);
+  for (size_t i = 0; i < n; i++) {
+    zeros[i].assign((size_t)steps * std::max(e->members[i]->ns, 1), 0.0);
+    psi[i] = zeros[i].data();
+  }
+  const int variant0 = e->variant;
+  const bool graph0 = e->use_graph;
+  e->use_graph = false;
+  int best = variant0;
+  double best_t = 1e300;
+  for (int v : cands) {
+    if (ens_set_variant(e, v)) continue;
+    std::vector<double> t;
+    int ierr = SE2W_OK;
+    for (int r = 0; r < reps + 1 && !ierr; r++) {  // first repetition is a warm-up
+      ierr = ens_run(e, steps, dts.data(), psi.data());
+      if (!ierr && r > 0) t.push_back(e->step_ms);
+    }
+    if (ierr || t.empty()) continue;
+    std::sort(t.begin(), t.end());
+    const double med = t[t.size() / 2];
+    if (med < best_t) { best_t = med; best = v; }
+  }
+  e->use_graph = graph0;
+  int ierr = SE2W_OK;
+  for (size_t i = 0; i < n; i++) {
+    int r = snapshot_restore(e->members[i], sn[i]);
+    if (r && !ierr) ierr = r;
+  }
+  if (ierr) return ierr;
+  if ((ierr = ens_set_variant(e, best))) return ierr;
+  if (best_variant) *best_variant = best;
+  if (best_ms) *best_ms = best_t;
+  return SE2W_OK;
+}
+
+int se2w_ensemble_get_member(se2w_ensemble *e, int32_t i, se2w_ctx **member) {
+  if (!e || !member) return fail(SE2W_ERR_ARG, "null argument");
+  if (i < 0 || i >= (int)e->members.size()) return fail(SE2W_ERR_ARG, "member %d out of range", i);
+  *member = e->members[i];
+  return SE2W_OK;
+}
+
+int se2w_ensemble_destroy(se2w_ensemble *e) {
+  if (!e) return SE2W_OK;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  ens_drop_graph(e);
+  dev_free(e->d_prms);
+  void *t = e->d_tile_off;
+  dev_free(t);
+  if (e->ev0) hipEventDestroy(e->ev0);
+  if (e->ev1) hipEventDestroy(e->ev1);
+  if (e->ev2) hipEventDestroy(e->ev2);
+  if (e->stream) hipStreamDestroy(e->stream);
+  delete e;
   return SE2W_OK;
 }
 

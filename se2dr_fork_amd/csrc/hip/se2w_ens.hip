@@ -1,0 +1,130 @@
+// se2w_ens.hip -- batched (ensemble) step kernels: a persistent grid walks the
+// tiles of all members; each workgroup runs the single-instance tile bodies
+// with the step parameters of the member that owns the tile.
+#include "se2w_ens.hpp"
+
+#include "se2w.h"
+#include "se2w_kernels_reg.hpp"
+
+namespace se2w {
+
+// Global tile t of the ensemble -> (member, member tile), walked by a
+// persistent grid.  The eight XCDs (workgroup b runs on XCD b % 8) each take
+// one contiguous eighth of the tile range, so neighbouring tiles of one
+// member, which re-read each other's halo elements, share an L2.
+template <typename R, typename Body>
+__device__ __forceinline__ void ens_loop(const EnsArgs<R> &a, Body body) {
+  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = gridDim.x >> 3;
+  const int per = (a.total + 7) >> 3;
+  const int t0 = xcd * per, t1 = min(t0 + per, a.total);
+  bool first = true;
+  for (int t = t0 + lb; t < t1; t += nbx) {
+    int lo = 0, hi = a.n - 1;  // largest i with tile_off[i] <= t
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.tile_off[mid] <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const int i = __builtin_amdgcn_readfirstlane(lo);
+    StepParams<R> prm = a.prms[2 * i + (a.step & 1)];
+    prm.step = a.step;
+    prm.last = a.last;
+    if (!first) __syncthreads();  // the previous tile's last phase has read its LDS
+    first = false;
+    body(prm, t - a.tile_off[i]);
+  }
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void ens_tile_kernel(EnsArgs<typename RealOf<MODE>::T> a) {
+  ens_loop(a, [](const StepParams<typename RealOf<MODE>::T> &prm, int tile) { tile_body<P, MODE, true>(prm, tile); });
+}
+
+template <int P, int MODE, int EX, int EY, int WAVES>
+__global__ __launch_bounds__(EX * EY) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+void ens_reg_kernel(EnsArgs<float> a) {
+  ens_loop(a, [](const StepParams<float> &prm, int tile) { reg_body<P, MODE, true, EX, EY>(prm, tile); });
+}
+
+namespace {
+
+template <int P, int MODE, int S>
+int reg_dispatch(int sched, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
+  if constexpr (S < kNumRegSchedules) {
+    if constexpr (reg_schedule_built(P, S, true)) {
+      if (sched == S) {
+        constexpr RegSchedule r = kRegSchedules[S];
+        auto k = ens_reg_kernel<P, MODE, r.ex, r.ey, r.waves>;
+        if (!args) {
+          int n = 0;
+          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, r.ex * r.ey, 0) != hipSuccess) return -1;
+          *bpc = n;
+          *thr = r.ex * r.ey;
+          return 0;
+        }
+        hipLaunchKernelGGL(k, dim3(grid), dim3(r.ex * r.ey), 0, s, *(const EnsArgs<float> *)args);
+        return 0;
+      }
+    }
+    return reg_dispatch<P, MODE, S + 1>(sched, args, grid, s, bpc, thr);
+  }
+  return -1;
+}
+
+template <int P, int MODE>
+int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
+  using R = typename RealOf<MODE>::T;
+  if (variant == SE2W_KERNEL_LDS) {
+    auto k = ens_tile_kernel<P, MODE>;
+    if (!args) {
+      int n = 0;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kThreads, 0) != hipSuccess) return -1;
+      *bpc = n;
+      *thr = kThreads;
+      return 0;
+    }
+    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, s, *(const EnsArgs<R> *)args);
+    return 0;
+  }
+  if constexpr (MODE != MODE_F64 && P >= 2 && P <= 6)
+    return reg_dispatch<P, MODE, 0>(variant - SE2W_KERNEL_REG, args, grid, s, bpc, thr);
+  return -1;
+}
+
+template <int P>
+int dispatch_p(int mode, int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
+  switch (mode) {
+    case SE2W_FP64: return dispatch_pm<P, MODE_F64>(variant, args, grid, s, bpc, thr);
+    case SE2W_FP32: return dispatch_pm<P, MODE_F32>(variant, args, grid, s, bpc, thr);
+    case SE2W_FP32_KAHAN: return dispatch_pm<P, MODE_F32K>(variant, args, grid, s, bpc, thr);
+    default: return -1;
+  }
+}
+
+int dispatch(int p, int mode, int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
+  switch (p) {
+    case 1: return dispatch_p<1>(mode, variant, args, grid, s, bpc, thr);
+    case 2: return dispatch_p<2>(mode, variant, args, grid, s, bpc, thr);
+    case 3: return dispatch_p<3>(mode, variant, args, grid, s, bpc, thr);
+    case 4: return dispatch_p<4>(mode, variant, args, grid, s, bpc, thr);
+    case 5: return dispatch_p<5>(mode, variant, args, grid, s, bpc, thr);
+    case 6: return dispatch_p<6>(mode, variant, args, grid, s, bpc, thr);
+    case 7: return dispatch_p<7>(mode, variant, args, grid, s, bpc, thr);
+    case 8: return dispatch_p<8>(mode, variant, args, grid, s, bpc, thr);
+    default: return -1;
+  }
+}
+
+}  // namespace
+
+int ens_launch(int p, int mode, int variant, const void *args, int grid, hipStream_t s) {
+  if (!args || grid < 8 || grid % 8) return -1;
+  int bpc = 0, thr = 0;
+  return dispatch(p, mode, variant, args, grid, s, &bpc, &thr);
+}
+
+int ens_occupancy(int p, int mode, int variant, int *blocks_per_cu, int *threads) {
+  return dispatch(p, mode, variant, nullptr, 0, nullptr, blocks_per_cu, threads);
+}
+
+}  // namespace se2w

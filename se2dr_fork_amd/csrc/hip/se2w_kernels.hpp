@@ -376,8 +376,12 @@ __device__ __forceinline__ void node_update(const StepParams<typename RealOf<MOD
 // Tile kernel.  FUSED: element operator + update in one pass (one launch per
 // step).  !FUSED: element operator only, f written for the update kernel.
 // ---------------------------------------------------------------------------
+// One workgroup processes one tile (all threads call it with the same tile).
+// A workgroup that calls it for several tiles in a row must put a barrier
+// between the calls: the last phase of one call reads `su`, which the first
+// phase of the next call overwrites.
 template <int P, int MODE, bool FUSED>
-__global__ __launch_bounds__(kThreads) void tile_kernel(StepParams<typename RealOf<MODE>::T> prm) {
+__device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>::T> &prm, int tile) {
   using R = typename RealOf<MODE>::T;
   using R2 = typename V2<R>::T;
   constexpr int NB = P + 1;
@@ -388,9 +392,6 @@ __global__ __launch_bounds__(kThreads) void tile_kernel(StepParams<typename Real
   __shared__ R ss[3][QY][QX];
   __shared__ R sD[NB * NB], sW[NB * NB];
 
-  const int ntiles = prm.tiles_x * prm.tiles_y;
-  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
-  if (tile >= ntiles) return;  // uniform per workgroup
   const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
   const int ex0 = tx * TEX, ey0 = ty * TEY;
   const int gx0 = (ex0 - 1) * P, gy0 = (ey0 - 1) * P;
@@ -492,6 +493,13 @@ __global__ __launch_bounds__(kThreads) void tile_kernel(StepParams<typename Real
       prm.f[(long)gy * prm.pitch + gx] = R2{fx, fy};
     }
   }
+}
+
+template <int P, int MODE, bool FUSED>
+__global__ __launch_bounds__(kThreads) void tile_kernel(StepParams<typename RealOf<MODE>::T> prm) {
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
+  tile_body<P, MODE, FUSED>(prm, tile);
 }
 
 // Update kernel of the split schedule: one workgroup per tile of owned nodes.

@@ -31,9 +31,16 @@ constexpr RegSchedule kRegSchedules[] = {{32, 4, 0}, {16, 4, 0}, {32, 2, 0}, {16
                                          {32, 4, 3}, {16, 16, 0}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
-template <int P, int MODE, bool FUSED, int EX, int EY, int WAVES>
-__global__ __launch_bounds__(EX * EY) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
-void reg_kernel(StepParams<float> prm) {
+// Schedules instantiated for (p, fused): all of them for p = 4, the first
+// three for the other degrees, the first one for the split schedule's
+// element kernel.
+constexpr bool reg_schedule_built(int P, int S, bool fused) { return fused ? (P == 4 || S < 3) : S == 0; }
+
+// One workgroup of EX x EY threads processes one tile of (EX-1) x (EY-1)
+// elements (all threads call it with the same tile).  A workgroup that calls
+// it for several tiles in a row puts a barrier between the calls.
+template <int P, int MODE, bool FUSED, int EX, int EY>
+__device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile) {
   static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
   constexpr int NB = P + 1;
   constexpr int TEX = EX - 1, TEY = EY - 1;
@@ -42,9 +49,6 @@ void reg_kernel(StepParams<float> prm) {
   // nodal forces of the owned nodes; column I stored at I + I/P
   __shared__ float2 sf[TEY * P + 1][TEX * P + 1 + TEX + 1];
 
-  const int ntiles = prm.tiles_x * prm.tiles_y;
-  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
-  if (tile >= ntiles) return;
   const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
   const int ex0 = tx * TEX, ey0 = ty * TEY;
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
@@ -210,6 +214,14 @@ void reg_kernel(StepParams<float> prm) {
       }
     }
   }
+}
+
+template <int P, int MODE, bool FUSED, int EX, int EY, int WAVES>
+__global__ __launch_bounds__(EX * EY) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+void reg_kernel(StepParams<float> prm) {
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
+  reg_body<P, MODE, FUSED, EX, EY>(prm, tile);
 }
 
 }  // namespace se2w

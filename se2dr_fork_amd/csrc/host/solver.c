@@ -155,3 +155,50 @@ int se2w_solver_get_trace(SE2WSolver sv, double *times, double *trace)
   if (trace && nr > 0 && sv->last_n) memcpy(trace, sv->trace, sizeof(double) * (size_t)sv->last_n * nr * 7);
   return 0;
 }
+
+/* Advances n solvers together: the host part of se2w_solver_advance for each
+ * (accumulated time, source-time table), one batched GPU step per step for all
+ * of them (se2w_ensemble_run), then each solver's receiver rows.  ens must have
+ * been created from the solvers' contexts in the same order. */
+int se2w_solvers_advance(int n, SE2WSolver *sv, se2w_ensemble *ens, int nsteps, const double *dt)
+{
+  const double **psi = NULL;
+  int i, k, s, ierr;
+  if (n < 1 || !sv || !ens || !dt || nsteps < 1) return host_fail(SE2W_HOST_ERR_ARG, "bad batched advance arguments");
+  for (i = 0; i < n; i++) {
+    se2w_ctx *c = NULL;
+    if (!sv[i]) return host_fail(SE2W_HOST_ERR_ARG, "solver %d is null", i);
+    if ((ierr = se2w_ensemble_get_member(ens, i, &c))) return gpu_fail(ierr);
+    if (c != sv[i]->ctx) return host_fail(SE2W_HOST_ERR_ARG, "ensemble member %d is not solver %d's context", i, i);
+  }
+  psi = (const double **)calloc((size_t)n, sizeof(double *));
+  if (!psi) return host_fail(SE2W_HOST_ERR_MEM, "out of memory");
+  for (i = 0; i < n; i++) {
+    struct se2w_solver *v = sv[i];
+    const int nr = v->recv ? v->recv->nr : 0;
+    double t = v->time;
+    if ((ierr = grow(&v->times, &v->times_cap, (size_t)nsteps))) goto done;
+    if (v->ns > 0 && (ierr = grow(&v->psi, &v->psi_cap, (size_t)nsteps * v->ns))) goto done;
+    if (nr > 0 && (ierr = grow(&v->trace, &v->trace_cap, (size_t)nsteps * nr * 7))) goto done;
+    for (k = 0; k < nsteps; k++) {
+      t = t + dt[i];
+      v->times[k] = t;
+      for (s = 0; s < v->ns; s++)
+        if ((ierr = se2w_stf_evaluate(v->stf[s], t, &v->psi[(size_t)k * v->ns + s]))) goto done;
+    }
+    psi[i] = v->ns > 0 ? v->psi : NULL;
+  }
+  if ((ierr = se2w_ensemble_run(ens, nsteps, dt, psi))) { ierr = gpu_fail(ierr); goto done; }
+  for (i = 0; i < n; i++) {
+    struct se2w_solver *v = sv[i];
+    const int nr = v->recv ? v->recv->nr : 0;
+    if (nr > 0 && (ierr = se2w_get_receivers(v->ctx, v->trace))) { ierr = gpu_fail(ierr); goto done; }
+    v->time = v->times[nsteps - 1];
+    v->steps += nsteps;
+    v->last_n = nsteps;
+  }
+  ierr = 0;
+done:
+  free(psi);
+  return ierr;
+}
