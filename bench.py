@@ -94,6 +94,14 @@ def kernel_signature(variant, p, mode):
     return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}>"
 
 
+def ens_kernel_signature(variant, p, mode):
+    """The batched (ensemble) step kernel's template signature."""
+    if variant == 0:
+        return f"ens_tile_kernel<{p}, {mode}>"
+    ex, ey, w = REG_SCHEDULES[variant - 1]
+    return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}>"
+
+
 def kernel_name(variant, p, mode):
     what = ("LDS-staged tile" if variant == 0 else
             "thread per element, %dx%d element threads" % REG_SCHEDULES[variant - 1][:2])
@@ -154,6 +162,44 @@ def cpu_baseline(sc, dt, seconds_target=15.0):
                       f"oracle/ref_cpu.c, {cores} OpenMP threads): {n}-step run minus 1-step run = {el:.1f} s"}
 
 
+def ensemble_bench(S, SC, n=256, nt=1500):
+    """The uneven ensemble workload: n instances of mx, my ~ U{64..192}, p = 6,
+    FP32, per-cell random materials, one Ricker source, 8 receivers, nt steps
+    each; all advanced together by one batched launch per step (HIP graph),
+    against the same n instances solved one after another."""
+    scs = [SC.config3_instance(i, nt=nt) for i in range(n)]
+    nodes = sum(sc.nnodes for sc in scs)
+    builts = [SC.build(sc, precision=S.FP32) for sc in scs]
+    ens = S.Ensemble([b.solver for b in builts])
+    best, _ = ens.autotune(steps=10, reps=2)
+    dts = [b.dt for b in builts]
+    t0 = time.perf_counter()
+    res = ens.advance(nt, dts)
+    t_ens = time.perf_counter() - t0
+    run_ms, step_ms = ens.timing()
+    tiles, grid = ens.launch()
+    ens.close()
+    for b in builts:
+        b.solver.close()
+    t_seq, errs = 0.0, []
+    for i, sc in enumerate(scs):
+        b = SC.build(sc, precision=S.FP32)
+        t0 = time.perf_counter()
+        _, tr = b.solver.advance(nt, b.dt)
+        t_seq += time.perf_counter() - t0
+        errs.append(rel_l2(tr[:, :, :4], res[i][1][:, :, :4]))
+        b.solver.close()
+    return {"workload": f"{n} instances, mx, my ~ U{{64..192}}, p=6, FP32, per-cell random material, "
+                        f"1 Ricker source (f0 ~ U[5,20] Hz), 8 receivers, {nt} steps each",
+            "instances": n, "nodes": nodes, "steps": nt,
+            "solves_per_s": n / t_ens, "sequential_solves_per_s": n / t_seq, "speedup_vs_sequential": t_seq / t_ens,
+            "ensemble_s": t_ens, "sequential_s": t_seq, "batched_step_ms": step_ms,
+            "node_updates_per_s": nodes * nt / t_ens,
+            "kernel": ens_kernel_signature(best, 6, 1),
+            "tiles": tiles, "grid": grid,
+            "max_rel_l2_u_v_vs_sequential": max(errs)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +211,9 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the device copy-bandwidth measurement")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=1500)
     ap.add_argument("--precisions", default="fp32k,fp32,fp64")
     ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
                     help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
@@ -228,6 +277,8 @@ def main():
     }
     if not args.no_copy:
         out["stream_copy_GBs"] = stream_copy_gbs()
+    if not args.no_ensemble:
+        out["ensemble"] = ensemble_bench(S, SC, args.ensemble_members, args.ensemble_steps)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"])
     else:
