@@ -93,6 +93,18 @@ def _declare(hip, host):
     hip.se2w_ensemble_get_launch.argtypes = [v, _ip, _ip]
     hip.se2w_ensemble_get_member.argtypes = [v, i32, C.POINTER(v)]
     hip.se2w_ensemble_destroy.argtypes = [v]
+    fp = C.POINTER(C.c_float)
+    hip.se2w_mlp_create.argtypes = [i32, _ip, i32, C.c_uint64, i32, C.POINTER(v)]
+    hip.se2w_mlp_param_count.argtypes = [v, C.POINTER(C.c_int64)]
+    hip.se2w_mlp_get_params.argtypes = [v, fp]
+    hip.se2w_mlp_set_params.argtypes = [v, fp]
+    hip.se2w_mlp_forward.argtypes = [v, i32, fp, fp]
+    hip.se2w_mlp_gradient.argtypes = [v, i32, fp, fp, fp, fp]
+    hip.se2w_mlp_set_data.argtypes = [v, i32, fp, fp]
+    hip.se2w_mlp_train.argtypes = [v, i32, C.c_float, C.c_uint64, fp]
+    hip.se2w_mlp_adam_step.argtypes = [v, fp, C.c_float]
+    hip.se2w_mlp_get_timing.argtypes = [v, _dp, _dp]
+    hip.se2w_mlp_destroy.argtypes = [v]
     host.se2w_host_last_error.restype = C.c_char_p
     host.se2w_gll.argtypes = [C.c_int, _dp, _dp]
     host.se2w_basis1d.argtypes = [C.c_int, C.c_int, _dp, _dp]
@@ -573,6 +585,86 @@ class Ensemble:
     def close(self):
         if self.h:
             self._hip.se2w_ensemble_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _fptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _f32(a, shape=None) -> np.ndarray:
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a if shape is None else a.reshape(shape)
+
+
+class Mlp:
+    """The surrogate network on the GPU (include/se2w_mlp.h): BF16 MFMA
+    products with FP32 accumulation, FP32 master weights, Adam."""
+
+    def __init__(self, widths: Sequence[int], batch=256, seed=0, device=0):
+        self._hip, _ = libraries()
+        self.widths = [int(w) for w in widths]
+        self.batch = batch
+        w = np.array(self.widths, dtype=np.int32)
+        self.h = C.c_void_p()
+        _gchk(self._hip.se2w_mlp_create(len(w) - 1, _ptr(w, _ip), batch, C.c_uint64(seed), device, C.byref(self.h)))
+        n = C.c_int64()
+        _gchk(self._hip.se2w_mlp_param_count(self.h, C.byref(n)))
+        self.nparams = n.value
+
+    def params(self) -> np.ndarray:
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _gchk(self._hip.se2w_mlp_get_params(self.h, _fptr(p)))
+        return p
+
+    def set_params(self, p):
+        p = _f32(p)
+        assert p.size == self.nparams
+        _gchk(self._hip.se2w_mlp_set_params(self.h, _fptr(p)))
+
+    def forward(self, x) -> np.ndarray:
+        x = _f32(x, (-1, self.widths[0]))
+        y = np.zeros((x.shape[0], self.widths[-1]), dtype=np.float32)
+        _gchk(self._hip.se2w_mlp_forward(self.h, x.shape[0], _fptr(x), _fptr(y)))
+        return y
+
+    def gradient(self, x, target):
+        x = _f32(x, (-1, self.widths[0]))
+        t = _f32(target, (-1, self.widths[-1]))
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_float()
+        _gchk(self._hip.se2w_mlp_gradient(self.h, x.shape[0], _fptr(x), _fptr(t), C.byref(loss), _fptr(g)))
+        return loss.value, g
+
+    def adam_step(self, grad, lr=1e-3):
+        g = _f32(grad)
+        _gchk(self._hip.se2w_mlp_adam_step(self.h, _fptr(g), lr))
+
+    def set_data(self, x, target):
+        x = _f32(x, (-1, self.widths[0]))
+        t = _f32(target, (-1, self.widths[-1]))
+        _gchk(self._hip.se2w_mlp_set_data(self.h, x.shape[0], _fptr(x), _fptr(t)))
+
+    def train(self, epochs: int, lr=1e-3, seed=0) -> np.ndarray:
+        losses = np.zeros(epochs, dtype=np.float32)
+        _gchk(self._hip.se2w_mlp_train(self.h, epochs, lr, C.c_uint64(seed), _fptr(losses)))
+        return losses
+
+    def timing(self):
+        """(last training in ms, last forward in ms)."""
+        a, b = C.c_double(), C.c_double()
+        _gchk(self._hip.se2w_mlp_get_timing(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def close(self):
+        if self.h:
+            self._hip.se2w_mlp_destroy(self.h)
             self.h = C.c_void_p()
 
     def __del__(self):

@@ -442,6 +442,14 @@ int run_mode(se2w_ctx *c, int nsteps, double dt) {
 
 }  // namespace
 
+namespace se2w {
+// Error message of the library's other translation units (se2w_last_error).
+int set_error(int code, const char *msg) {
+  snprintf(g_err, sizeof(g_err), "%s", msg);
+  return code;
+}
+}  // namespace se2w
+
 extern "C" {
 
 int se2w_version(void) { return 1; }

@@ -12,7 +12,7 @@ import pytest
 
 from conftest import ROOT
 
-HEADERS = {"se2w.h": "libse2w_hip.so", "se2wave.h": "libse2w_host.so"}
+HEADERS = {"se2w.h": "libse2w_hip.so", "se2w_mlp.h": "libse2w_hip.so", "se2wave.h": "libse2w_host.so"}
 
 
 def declared_functions(header):
@@ -46,7 +46,7 @@ def test_error_strings(se2w):
 
 def test_c_headers_compile_as_c99():
     """The ABI headers are plain C (no torch or C++ types in the signatures)."""
-    src = '#include "se2w.h"\n#include "se2wave.h"\nint main(void){return 0;}\n'
+    src = '#include "se2w.h"\n#include "se2w_mlp.h"\n#include "se2wave.h"\nint main(void){return 0;}\n'
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         "-x", "c", "-"], input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
