@@ -200,6 +200,53 @@ def ensemble_bench(S, SC, n=256, nt=1500):
             "max_rel_l2_u_v_vs_sequential": max(errs)}
 
 
+def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
+    """The learned surrogate: training data from the batched solver (96^2,
+    p=4, FP32), the MLP trained on the GPU, its held-out error against the
+    solver and its time per answer against the solver's time per solve."""
+    from se2dr_fork_amd import surrogate as SG
+
+    tr = SG.generate(range(ntrain))
+    te = SG.generate(range(100000, 100000 + ntest))
+    sg = SG.Surrogate()
+    losses = sg.fit(tr, epochs=epochs, lr=1e-3)
+    train_ms = sg.mlp.timing()[0]
+    sg.predict(te.x)  # warm-up
+    t0 = time.perf_counter()
+    yp = sg.predict(te.x)
+    t_wall = time.perf_counter() - t0
+    fwd_ms = sg.mlp.timing()[1]
+    sg.close()
+    # the plain solver one instance at a time (what the surrogate replaces)
+    nseq = 8
+    t_seq = 0.0
+    for k in range(nseq):
+        sc, _ = SG.instance(100000 + k)
+        b = SC.build(sc, precision=S.FP32)
+        t0 = time.perf_counter()
+        b.solver.advance(SG.STRIDE * SG.NSAMP, b.dt)
+        t_seq += time.perf_counter() - t0
+        b.solver.close()
+    per_solve_seq = t_seq / nseq
+    per_solve_batched = te.seconds / ntest
+    per_answer = t_wall / ntest
+    d = ((te.x[:, None, :2] - tr.x[None, :, :2]) ** 2).sum(-1)
+    return {"workload": f"{ntrain} training + {ntest} held-out FP32 solves, 96x96 elements, p=4, 1 Hz Ricker point "
+                        f"force, Vp = 5000 (1 + d), d ~ N(0, 5%) on 8x8 blocks; 4 receivers x (vx, vy) x 512 samples",
+            "network": "66-1024-1024-1024-4096 GELU, BF16 MFMA / FP32 accumulate, Adam lr 1e-3 cosine-decayed, batch 256, "
+                       f"{epochs} epochs, mirror-symmetry augmentation (4x)",
+            "held_out_rel_l2": SG.rel_l2(yp, te.y),
+            "held_out_median_sample_rel_l2": float(np.median(SG.per_sample_rel_l2(yp, te.y))),
+            "mean_predictor_rel_l2": SG.rel_l2(np.broadcast_to(tr.y.mean(axis=0), te.y.shape), te.y),
+            "nearest_source_predictor_rel_l2": SG.rel_l2(tr.y[np.argmin(d, axis=1)], te.y),
+            "final_training_loss": float(losses[-1]),
+            "data_generation_s": tr.seconds + te.seconds, "training_s": train_ms / 1e3,
+            "surrogate_s_per_answer_wall": per_answer, "surrogate_device_ms_for_held_out": fwd_ms,
+            "solver_s_per_solve_sequential": per_solve_seq, "solver_s_per_solve_batched": per_solve_batched,
+            "speedup_vs_sequential_solver": per_solve_seq / per_answer,
+            "speedup_vs_batched_solver": per_solve_batched / per_answer}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -213,6 +260,7 @@ def main():
     ap.add_argument("--no-copy", action="store_true", help="skip the device copy-bandwidth measurement")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
     ap.add_argument("--ensemble-steps", type=int, default=1500)
     ap.add_argument("--precisions", default="fp32k,fp32,fp64")
     ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
@@ -279,6 +327,8 @@ def main():
         out["stream_copy_GBs"] = stream_copy_gbs()
     if not args.no_ensemble:
         out["ensemble"] = ensemble_bench(S, SC, args.ensemble_members, args.ensemble_steps)
+    if not args.no_surrogate:
+        out["surrogate"] = surrogate_bench(S, SC)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"])
     else:

@@ -46,6 +46,13 @@ int se2w_mlp_set_data(se2w_mlp *mlp, int32_t n, const float *x, const float *tar
  * full batches (a remainder smaller than a batch is skipped); epoch_loss
  * [epochs] receives each epoch's mean batch loss (may be NULL). */
 int se2w_mlp_train(se2w_mlp *mlp, int32_t epochs, float lr, uint64_t shuffle_seed, float *epoch_loss);
+/* Learning rate of se2w_mlp_train: cosine = 0 (default) keeps lr constant;
+ * cosine = 1 decays it as lr (1 + cos(pi s / S)) / 2 over the S batch steps
+ * of the call (Loshchilov & Hutter 2017, without restarts). */
+int se2w_mlp_set_lr_schedule(se2w_mlp *mlp, int32_t cosine);
+/* Decoupled weight decay of the Adam update (AdamW, Loshchilov & Hutter
+ * 2019): w -= lr (mhat / (sqrt(vhat) + eps) + wd w).  Default 0 (Adam). */
+int se2w_mlp_set_weight_decay(se2w_mlp *mlp, float wd);
 /* One Adam step with the given gradient (same layout as the parameters). */
 int se2w_mlp_adam_step(se2w_mlp *mlp, const float *grad, float lr);
 /* Wall time of the last se2w_mlp_train and of the last se2w_mlp_forward's

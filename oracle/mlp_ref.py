@@ -80,13 +80,14 @@ def loss_and_gradient(params, widths, x, target, bf=True):
     return loss, pack(grads)
 
 
-def adam_step(params, m, v, grad, t, lr, b1=0.9, b2=0.999, eps=1e-8):
-    """One Adam step (Kingma & Ba 2015, Algorithm 1); returns new (params, m, v)."""
+def adam_step(params, m, v, grad, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    """One Adam step (Kingma & Ba 2015, Algorithm 1), with decoupled weight
+    decay wd (AdamW, Loshchilov & Hutter 2019); returns new (params, m, v)."""
     m = b1 * m + (1 - b1) * grad
     v = b2 * v + (1 - b2) * grad * grad
     mh = m / (1 - b1 ** t)
     vh = v / (1 - b2 ** t)
-    return params - lr * mh / (np.sqrt(vh) + eps), m, v
+    return params - lr * (mh / (np.sqrt(vh) + eps) + wd * params), m, v
 
 
 def init_params(widths, seed):

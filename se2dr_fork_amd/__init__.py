@@ -103,6 +103,8 @@ def _declare(hip, host):
     hip.se2w_mlp_set_data.argtypes = [v, i32, fp, fp]
     hip.se2w_mlp_train.argtypes = [v, i32, C.c_float, C.c_uint64, fp]
     hip.se2w_mlp_adam_step.argtypes = [v, fp, C.c_float]
+    hip.se2w_mlp_set_weight_decay.argtypes = [v, C.c_float]
+    hip.se2w_mlp_set_lr_schedule.argtypes = [v, i32]
     hip.se2w_mlp_get_timing.argtypes = [v, _dp, _dp]
     hip.se2w_mlp_destroy.argtypes = [v]
     host.se2w_host_last_error.restype = C.c_char_p
@@ -641,6 +643,12 @@ class Mlp:
         loss = C.c_float()
         _gchk(self._hip.se2w_mlp_gradient(self.h, x.shape[0], _fptr(x), _fptr(t), C.byref(loss), _fptr(g)))
         return loss.value, g
+
+    def set_lr_schedule(self, cosine: bool):
+        _gchk(self._hip.se2w_mlp_set_lr_schedule(self.h, 1 if cosine else 0))
+
+    def set_weight_decay(self, wd: float):
+        _gchk(self._hip.se2w_mlp_set_weight_decay(self.h, wd))
 
     def adam_step(self, grad, lr=1e-3):
         g = _f32(grad)

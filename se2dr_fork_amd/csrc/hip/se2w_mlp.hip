@@ -188,12 +188,13 @@ __global__ void k_bias_grad(const bf16 *dZt, int B, int N, float *gb) {
 
 struct AdamArgs {
   float lr, b1, b2, eps, bc1, bc2;  // bc = 1 - beta^t
+  float wd;                         // decoupled weight decay (AdamW); 0 = Adam
 };
 
 __device__ __forceinline__ float adam_update(float &w, float &m, float &v, float g, const AdamArgs &a) {
   m = a.b1 * m + (1.f - a.b1) * g;
   v = a.b2 * v + (1.f - a.b2) * g * g;
-  w -= a.lr * (m / a.bc1) / (sqrtf(v / a.bc2) + a.eps);
+  w -= a.lr * ((m / a.bc1) / (sqrtf(v / a.bc2) + a.eps) + a.wd * w);
   return w;
 }
 
@@ -269,6 +270,8 @@ struct se2w_mlp {
   hipStream_t stream = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   long t = 0;  // Adam steps taken
+  float weight_decay = 0.f;
+  int cosine = 0;  // cosine learning-rate decay over each se2w_mlp_train call
   double train_ms = 0.0, fwd_ms = 0.0;
   std::vector<void *> allocs;
 };
@@ -376,6 +379,7 @@ int adam(se2w_mlp *m, float lr) {
   a.b1 = 0.9f;
   a.b2 = 0.999f;
   a.eps = 1e-8f;
+  a.wd = m->weight_decay;
   a.bc1 = (float)(1.0 - std::pow(0.9, (double)m->t));
   a.bc2 = (float)(1.0 - std::pow(0.999, (double)m->t));
   for (MlpLayer &Ly : m->L) {
@@ -652,7 +656,12 @@ int se2w_mlp_train(se2w_mlp *m, int32_t epochs, float lr, uint64_t shuffle_seed,
       m->acc = d_loss;
       ierr = backward(m, e, 1.0 / ((double)m->B * m->out * nb));
       m->acc = acc0;
-      if (!ierr) ierr = adam(m, lr);
+      if (!ierr) {
+        // cosine decay: lr_s = lr (1 + cos(pi s / S)) / 2 over the S steps of this call
+        const double frac = (double)(e * nb + k) / ((double)epochs * nb);
+        const float lr_s = m->cosine ? (float)(0.5 * lr * (1.0 + std::cos(3.14159265358979323846 * frac))) : lr;
+        ierr = adam(m, lr_s);
+      }
     }
   }
   if (!ierr) {
@@ -673,6 +682,18 @@ int se2w_mlp_train(se2w_mlp *m, int32_t epochs, float lr, uint64_t shuffle_seed,
   hipFree(d_perm);
   hipFree(d_loss);
   return ierr;
+}
+
+int se2w_mlp_set_lr_schedule(se2w_mlp *m, int32_t cosine) {
+  if (!m) return mfail(SE2W_ERR_ARG, "null argument");
+  m->cosine = cosine != 0;
+  return SE2W_OK;
+}
+
+int se2w_mlp_set_weight_decay(se2w_mlp *m, float wd) {
+  if (!m || !(wd >= 0.f)) return mfail(SE2W_ERR_ARG, "bad weight decay");
+  m->weight_decay = wd;
+  return SE2W_OK;
 }
 
 int se2w_mlp_get_timing(se2w_mlp *m, double *train_ms, double *forward_ms) {
