@@ -15,6 +15,7 @@
 #include "se2w.h"
 #include "se2w_kernels.hpp"
 #include "se2w_kernels_reg.hpp"
+#include "se2w_kernels_mfma.hpp"
 #include "se2w_ens.hpp"
 
 using namespace se2w;
@@ -192,6 +193,12 @@ int upload_real(se2w_ctx *c, void **dst, const double *src, size_t n) {
 // Register-blocked kernel availability: FP32 modes, 2 <= p <= 6.
 bool reg_available(int p, int mode) { return mode != SE2W_FP64 && p >= 2 && p <= 6; }
 
+// Matrix-core kernel availability: FP32 modes, 6 <= p <= 8, fused schedule.
+static_assert(SE2W_KERNEL_MFMA == SE2W_KERNEL_REG + kNumRegSchedules, "variant numbering");
+bool mfma_available(int p, int mode, int schedule) {
+  return mode != SE2W_FP64 && p >= 6 && p <= 8 && schedule == SE2W_SCHED_FUSED;
+}
+
 // variant = SE2W_KERNEL_REG + schedule index into kRegSchedules
 int reg_schedule_of(int variant) { return variant - SE2W_KERNEL_REG; }
 bool is_reg_variant(int variant) { return variant >= SE2W_KERNEL_REG && variant < SE2W_KERNEL_REG + kNumRegSchedules; }
@@ -211,6 +218,15 @@ void tile_shape(int p, int mode, int &tex, int &tey) {
     break;
   switch (p) { CASE(1) CASE(2) CASE(3) CASE(4) CASE(5) CASE(6) CASE(7) CASE(8) default: tex = tey = 0; }
 #undef CASE
+}
+
+void mfma_shape(int p, int &tex, int &tey) {
+  switch (p) {
+    case 6: tex = MfmaShape<6>::TEX; tey = MfmaShape<6>::TEY; break;
+    case 7: tex = MfmaShape<7>::TEX; tey = MfmaShape<7>::TEY; break;
+    case 8: tex = MfmaShape<8>::TEX; tey = MfmaShape<8>::TEY; break;
+    default: tex = tey = 0;
+  }
 }
 
 // Per-tile CSR of the items (sources or receivers) whose element touches
@@ -243,6 +259,7 @@ void build_tile_lists(const se2w_ctx *c, const std::vector<int> &elem, std::vect
 
 void set_tiling(se2w_ctx *c) {
   if (is_reg_variant(c->variant)) reg_shape(c->variant, c->tex, c->tey);
+  else if (c->variant == SE2W_KERNEL_MFMA) mfma_shape(c->p, c->tex, c->tey);
   else tile_shape(c->p, c->mode, c->tex, c->tey);
   c->tiles_x = (c->mx + c->tex - 1) / c->tex;
   c->tiles_y = (c->my + c->tey - 1) / c->tey;
@@ -314,6 +331,12 @@ void launch_element_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &p
   const dim3 grid(8 * c->tiles_per_xcd);
   if constexpr (MODE != MODE_F64 && P >= 2 && P <= 6) {
     if (is_reg_variant(c->variant) && launch_reg_s<P, MODE, FUSED, 0>(c, prm)) return;
+  }
+  if constexpr (MODE != MODE_F64 && P >= 6 && FUSED) {
+    if (c->variant == SE2W_KERNEL_MFMA) {
+      hipLaunchKernelGGL((mfma_kernel<P, MODE>), grid, dim3(kThreads), 0, c->stream, prm);
+      return;
+    }
   }
   hipLaunchKernelGGL((tile_kernel<P, MODE, FUSED>), grid, dim3(kThreads), 0, c->stream, prm);
 }
@@ -887,7 +910,10 @@ int se2w_get_timing(se2w_ctx *c, double *run_ms, double kernel_ms[2]) {
 
 int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
-  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant)) return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
+  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant) && variant != SE2W_KERNEL_MFMA)
+    return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
+  if (variant == SE2W_KERNEL_MFMA && !mfma_available(c->p, c->mode, c->schedule))
+    return fail(SE2W_ERR_UNSUPPORTED, "matrix-core kernel needs FP32, 6 <= p <= 8 and the fused schedule");
   if (is_reg_variant(variant) && !reg_available(c->p, c->mode))
     return fail(SE2W_ERR_UNSUPPORTED, "register-blocked kernel needs FP32 and 2 <= p <= 6");
   if (is_reg_variant(variant) && !reg_variant_built(c->p, variant, c->schedule))
@@ -909,7 +935,7 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   return SE2W_OK;
 }
 
-int se2w_kernel_count(void) { return SE2W_KERNEL_REG + kNumRegSchedules; }
+int se2w_kernel_count(void) { return SE2W_KERNEL_MFMA + 1; }
 
 // Autotuner: times every variant built for the context's (p, precision,
 // schedule) on the context's own mesh and keeps the fastest.  The state is
@@ -989,6 +1015,7 @@ int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_varian
   if (reg_available(c->p, c->mode))
     for (int k = 0; k < kNumRegSchedules; k++)
       if (reg_variant_built(c->p, SE2W_KERNEL_REG + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG + k);
+  if (mfma_available(c->p, c->mode, c->schedule)) cands.push_back(SE2W_KERNEL_MFMA);
   Snapshot sn;
   if (int ierr = snapshot_take(c, sn)) return ierr;
   const int variant0 = c->variant;
@@ -1269,7 +1296,7 @@ int se2w_ensemble_autotune(se2w_ensemble *e, int32_t steps, int32_t reps, int32_
   if (!e || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
   HIPCHK(hipSetDevice(e->device));
   std::vector<int> cands;
-  for (int v = 0; v < SE2W_KERNEL_REG + kNumRegSchedules; v++)
+  for (int v = 0; v <= SE2W_KERNEL_MFMA; v++)
     if (ens_variant_built(e, v)) cands.push_back(v);
   const size_t n = e->members.size();
   std::vector<Snapshot> sn(n);

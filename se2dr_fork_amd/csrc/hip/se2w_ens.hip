@@ -5,6 +5,7 @@
 
 #include "se2w.h"
 #include "se2w_kernels_reg.hpp"
+#include "se2w_kernels_mfma.hpp"
 
 namespace se2w {
 
@@ -46,6 +47,11 @@ void ens_reg_kernel(EnsArgs<float> a) {
   ens_loop(a, [](const StepParams<float> &prm, int tile) { reg_body<P, MODE, true, EX, EY>(prm, tile); });
 }
 
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void ens_mfma_kernel(EnsArgs<float> a) {
+  ens_loop(a, [](const StepParams<float> &prm, int tile) { mfma_body<P, MODE>(prm, tile); });
+}
+
 namespace {
 
 template <int P, int MODE, int S>
@@ -85,6 +91,20 @@ int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc
     }
     hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, s, *(const EnsArgs<R> *)args);
     return 0;
+  }
+  if constexpr (MODE != MODE_F64 && P >= 6) {
+    if (variant == SE2W_KERNEL_MFMA) {
+      auto k = ens_mfma_kernel<P, MODE>;
+      if (!args) {
+        int n = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kThreads, 0) != hipSuccess) return -1;
+        *bpc = n;
+        *thr = kThreads;
+        return 0;
+      }
+      hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, s, *(const EnsArgs<float> *)args);
+      return 0;
+    }
   }
   if constexpr (MODE != MODE_F64 && P >= 2 && P <= 6)
     return reg_dispatch<P, MODE, 0>(variant - SE2W_KERNEL_REG, args, grid, s, bpc, thr);

@@ -1,0 +1,240 @@
+// se2w_kernels_mfma.hpp -- fused FP32 step for high degrees (6 <= p <= 8)
+// with the element contractions on the matrix cores.
+//
+// The tile, its halo and the node update are those of tile_body; the eight
+// sum-factorised contractions of the element operator run as batched small
+// matrix products on v_mfma_f32_16x16x4_f32 (FP32 in, FP32 accumulate):
+//
+//   B1  dxi u (row, el, qi)  = sum_k D[qi][k] u(row, el P + k)     M = qi, N = 16 node rows
+//   B2  deta u (er, qj, col) = sum_k D[qj][k] u(er P + k, col)     M = qj, N = 16 node columns
+//   C1  R(el, i, qrow)      = sum_qi D[qi][i] S(qrow, el, qi)     M = i,  N = 16 quadrature rows
+//   C2  C(er, j, qcol)      = sum_qj D[qj][j] S(er, qj, qcol)     M = j,  N = 16 quadrature columns
+//
+// with K = the p+1 contraction points padded to 12 (three k-steps of 4).
+// Every node row or column of the tile is one N column of a product, so a
+// product covers whole rows of elements at once and the tiny D (or D^T) is
+// the constant A operand, held in three registers per lane.
+//
+// MFMA 16x16x4 f32 operand layout (lane l of 64): A[m][k] at m = l % 16,
+// k = l / 16; B[k][n] at k = l / 16, n = l % 16; result D[m][n] in
+// accumulator r at m = 4 (l / 16) + r, n = l % 16.
+#pragma once
+
+#include "se2w_kernels.hpp"
+
+namespace se2w {
+
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float mfma_f4;
+
+// Tile shapes (owned elements TEX x TEY) keeping the LDS of one workgroup
+// under 80 KB, so that two fit a CU.
+template <int P> struct MfmaShape;
+template <> struct MfmaShape<6> { static constexpr int TEX = 6, TEY = 5; };
+template <> struct MfmaShape<7> { static constexpr int TEX = 5, TEY = 5; };
+template <> struct MfmaShape<8> { static constexpr int TEX = 4, TEY = 4; };
+
+template <int P>
+struct MfmaLayout {
+  static constexpr int NB = P + 1;
+  static constexpr int TEX = MfmaShape<P>::TEX, TEY = MfmaShape<P>::TEY;
+  static constexpr int NXL = (TEX + 1) * P + 1, NYL = (TEY + 1) * P + 1;  // tile nodes (+ halo)
+  static constexpr int QX = (TEX + 1) * NB, QY = (TEY + 1) * NB;          // quadrature points (+ halo)
+  // LDS: the weighted stresses S, and a work area that holds first the
+  // derivatives (Dx, Dy), then the transposed contractions (R, C)
+  static constexpr int DXY = 2 * NYL * QX + 2 * QY * NXL;
+  static constexpr int SS = 3 * QY * QX;
+  static constexpr int RC = 4 * QX * QY;
+  static constexpr int WORK = DXY > RC ? DXY : RC;
+  // products per phase and per wave (4 waves)
+  static constexpr int JB1 = 2 * (TEX + 1) * ((NYL + 15) / 16);
+  static constexpr int JB2 = 2 * (TEY + 1) * ((NXL + 15) / 16);
+  static constexpr int JC1 = 2 * (TEX + 1) * ((QY + 15) / 16);
+  static constexpr int JC2 = 2 * (TEY + 1) * ((QX + 15) / 16);
+  static constexpr int JC = JC1 + JC2;
+  static constexpr int PTS = QX * QY;
+};
+
+template <int P, int MODE>
+__device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile) {
+  static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
+  using Lay = MfmaLayout<P>;
+  constexpr int NB = Lay::NB, TEX = Lay::TEX, TEY = Lay::TEY;
+  constexpr int NXL = Lay::NXL, NYL = Lay::NYL, QX = Lay::QX, QY = Lay::QY;
+  __shared__ float su[2][NYL][NXL];
+  __shared__ float S[Lay::SS];         // [3][QY][QX]: sxx, syy, sxy times detJ w_q
+  __shared__ float work[Lay::WORK];
+  float *Dx = work;                    // [2][NYL][QX]
+  float *Dy = work + 2 * NYL * QX;     // [2][QY][NXL]
+  float *Rr = work;                    // [2][QX][QY]
+  float *Cc = work + 2 * QX * QY;      // [2][QY][QX]
+
+  const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
+  const int ex0 = tx * TEX, ey0 = ty * TEY;
+  const int gx0 = (ex0 - 1) * P, gy0 = (ey0 - 1) * P;
+  const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+  const int lm = lane & 15, lk = lane >> 4;
+
+  // A operands: D[m][k] and D^T[m][k] = D[k][m] for the three k-steps
+  float Ad[3], At[3];
+#pragma unroll
+  for (int s = 0; s < 3; s++) {
+    const int k = 4 * s + lk;
+    Ad[s] = (lm < NB && k < NB) ? prm.D[lm * NB + k] : 0.f;
+    At[s] = (lm < NB && k < NB) ? prm.D[k * NB + lm] : 0.f;
+  }
+  // Phase A: displacement of the tile's nodes (+ halo) into LDS.
+  for (int k = tid; k < NXL * NYL; k += kThreads) {
+    const int ly = k / NXL, lx = k - ly * NXL;
+    const int gx = gx0 + lx, gy = gy0 + ly;
+    float2 uv = float2{0.f, 0.f};
+    if (gx >= 0 && gx < prm.nx && gy >= 0 && gy < prm.ny) uv = prm.u_in[(long)gy * prm.pitch + gx];
+    su[0][ly][lx] = uv.x;
+    su[1][ly][lx] = uv.y;
+  }
+  __syncthreads();
+  // B1: xi-derivatives along every node row, per element column.
+  for (int job = wave; job < Lay::JB1; job += 4) {
+    const int c = job & 1, rest = job >> 1;
+    const int el = rest % (TEX + 1), row0 = 16 * (rest / (TEX + 1));
+    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int row = row0 + lm;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      const int k = 4 * s + lk;
+      const float b = (k < NB && row < NYL) ? su[c][row][el * P + k] : 0.f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ad[s], b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int qi = 4 * lk + r;
+      if (qi < NB && row < NYL) Dx[(c * NYL + row) * QX + el * NB + qi] = acc[r];
+    }
+  }
+  // B2: eta-derivatives along every node column, per element row.
+  for (int job = wave; job < Lay::JB2; job += 4) {
+    const int c = job & 1, rest = job >> 1;
+    const int er = rest % (TEY + 1), col0 = 16 * (rest / (TEY + 1));
+    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+    const int col = col0 + lm;
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      const int k = 4 * s + lk;
+      const float b = (k < NB && col < NXL) ? su[c][er * P + k][col] : 0.f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ad[s], b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int qj = 4 * lk + r;
+      if (qj < NB && col < NXL) Dy[(c * QY + er * NB + qj) * NXL + col] = acc[r];
+    }
+  }
+  __syncthreads();
+  // B3: weighted stresses at the quadrature points.
+  for (int pnt = tid; pnt < Lay::PTS; pnt += kThreads) {
+    const int qrow = pnt / QX, qcol = pnt - qrow * QX;
+    const int er = qrow / NB, qj = qrow - er * NB, el = qcol / NB, qi = qcol - el * NB;
+    const int ex = ex0 - 1 + el, ey = ey0 - 1 + er;
+    float sxx = 0.f, syy = 0.f, sxy = 0.f;
+    if (ex >= 0 && ex < prm.mx && ey >= 0 && ey < prm.my) {
+      const int row = er * P + qj, col = el * P + qi;
+      const float dxux = Dx[(0 * NYL + row) * QX + qcol], dxuy = Dx[(1 * NYL + row) * QX + qcol];
+      const float dyux = Dy[(0 * QY + qrow) * NXL + col], dyuy = Dy[(1 * QY + qrow) * NXL + col];
+      const int e = ex + ey * prm.mx;
+      const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
+      const float iJ00 = 1.f / J00, iJ11 = 1.f / J11;
+      const float fac = (J00 * J11) * prm.w2[qi + qj * NB];
+      const float lam = prm.lambda[e], mu = prm.mu[e], c11 = 2.f * mu + lam;
+      const float e0 = iJ00 * dxux, e1 = iJ11 * dyuy, e2 = iJ11 * dyux + iJ00 * dxuy;
+      sxx = fac * (c11 * e0 + lam * e1);
+      syy = fac * (lam * e0 + c11 * e1);
+      sxy = fac * (mu * e2);
+    }
+    S[0 * QY * QX + pnt] = sxx;
+    S[1 * QY * QX + pnt] = syy;
+    S[2 * QY * QX + pnt] = sxy;
+  }
+  __syncthreads();  // S complete; the derivatives are dead, R and C reuse their space
+  // C1: row terms R(c, el, i, qrow); c = 0 from sxx (x force), c = 1 from sxy (y force)
+  for (int job = wave; job < Lay::JC1; job += 4) {
+    const int c = job & 1, rest = job >> 1;
+    const int el = rest % (TEX + 1), q0 = 16 * (rest / (TEX + 1));
+    const float *Src = S + (c == 0 ? 0 : 2) * QY * QX;
+    const int qrow = q0 + lm;
+    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      const int k = 4 * s + lk;
+      const float b = (k < NB && qrow < QY) ? Src[qrow * QX + el * NB + k] : 0.f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(At[s], b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int i = 4 * lk + r;
+      if (i < NB && qrow < QY) Rr[(c * QX + el * NB + i) * QY + qrow] = acc[r];
+    }
+  }
+  // C2: column terms C(c, er, j, qcol); c = 0 from sxy (x force), c = 1 from syy (y force)
+  for (int job = wave; job < Lay::JC2; job += 4) {
+    const int c = job & 1, rest = job >> 1;
+    const int er = rest % (TEY + 1), q0 = 16 * (rest / (TEY + 1));
+    const float *Src = S + (c == 0 ? 2 : 1) * QY * QX;
+    const int qcol = q0 + lm;
+    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+      const int k = 4 * s + lk;
+      const float b = (k < NB && qcol < QX) ? Src[(er * NB + k) * QX + qcol] : 0.f;
+      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(At[s], b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; r++) {
+      const int j = 4 * lk + r;
+      if (j < NB && qcol < QX) Cc[(c * QY + er * NB + j) * QX + qcol] = acc[r];
+    }
+  }
+  __syncthreads();
+  // D: owned nodes: element contributions in ascending element order, update.
+  const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
+  const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
+  const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
+  for (int k = tid; k < OX * OY; k += kThreads) {
+    const int oy = k / OX, ox = k - oy * OX;
+    const int gx = ex0 * P + ox, gy = ey0 * P + oy;
+    int exs[2], eys[2], nex = 0, ney = 0;
+    if (gx % P == 0) {
+      if (gx / P - 1 >= 0) exs[nex++] = gx / P - 1;
+      if (gx / P < prm.mx) exs[nex++] = gx / P;
+    } else {
+      exs[nex++] = gx / P;
+    }
+    if (gy % P == 0) {
+      if (gy / P - 1 >= 0) eys[ney++] = gy / P - 1;
+      if (gy / P < prm.my) eys[ney++] = gy / P;
+    } else {
+      eys[ney++] = gy / P;
+    }
+    float fx = 0.f, fy = 0.f;
+    for (int b = 0; b < ney; b++) {
+      for (int a = 0; a < nex; a++) {
+        const int ex = exs[a], ey = eys[b];
+        const int el = ex - ex0 + 1, er = ey - ey0 + 1;
+        const int i = gx - ex * P, j = gy - ey * P;
+        const float iJ00 = 1.f / (0.5f * prm.elem_dx[ex]), iJ11 = 1.f / (0.5f * prm.elem_dy[ey]);
+        const int qr = er * NB + j, qc = el * NB + i;
+        fx += -(iJ00 * Rr[(0 * QX + qc) * QY + qr] + iJ11 * Cc[(0 * QY + qr) * QX + qc]);
+        fy += -(iJ00 * Rr[(1 * QX + qc) * QY + qr] + iJ11 * Cc[(1 * QY + qr) * QX + qc]);
+      }
+    }
+    const int lx = gx - gx0, ly = gy - gy0;
+    node_update<P, MODE>(prm, tile, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
+  }
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void mfma_kernel(StepParams<float> prm) {
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
+  mfma_body<P, MODE>(prm, tile);
+}
+
+}  // namespace se2w

@@ -261,3 +261,27 @@ def test_gpu_error_paths():
         S.Context(4, 4, 3, precision=S.FP64).set_kernel(S.KERNEL_REG)
     with pytest.raises(S.Se2wError, match="p = 9"):
         S.Context(4, 4, 9)
+
+
+@pytest.mark.parametrize("p", [6, 7, 8])
+def test_mfma_kernel_matches_the_vector_kernels(p):
+    """Matrix-core contractions (FP32 in, FP32 accumulate) sum in another
+    order than the vector kernels: agreement to FP32 rounding, and the same
+    distance from the FP64 result."""
+    sc = SC.small(mx=23, my=17, p=p, nt=50, nsrc=2, nrec=5, seed=60 + p)  # partial edge tiles
+    _, _, r64, u64, _, _ = gpu_run(sc, precision=S.FP64)
+    for prec in [S.FP32, S.FP32_KAHAN]:
+        _, _, r0, u0, v0, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_LDS)
+        _, _, r1, u1, v1, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_MFMA)
+        assert rel_l2(u1, u0) < 2e-6 and rel_l2(v1, v0) < 2e-6
+        assert rel_l2(r1[..., :4], r64[..., :4]) < 5e-6
+        assert rel_l2(u1, u64) < 3 * max(rel_l2(u0, u64), 1e-7)
+
+
+def test_mfma_kernel_is_rejected_where_not_built():
+    b = SC.build(SC.small(mx=5, my=4, p=4), precision=S.FP32)
+    with pytest.raises(S.Se2wError, match="matrix-core"):
+        b.solver.ctx.set_kernel(S.KERNEL_MFMA)
+    b64 = SC.build(SC.small(mx=5, my=4, p=6), precision=S.FP64)
+    with pytest.raises(S.Se2wError, match="matrix-core"):
+        b64.solver.ctx.set_kernel(S.KERNEL_MFMA)
