@@ -90,6 +90,8 @@ def kernel_signature(variant, p, mode):
     """The step kernel's template signature as rocprofv3 prints it."""
     if variant == 0:
         return f"tile_kernel<{p}, {mode}, true>"
+    if variant == 9:
+        return f"mfma_kernel<{p}, {mode}>"
     ex, ey, w = REG_SCHEDULES[variant - 1]
     return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}>"
 
@@ -98,12 +100,14 @@ def ens_kernel_signature(variant, p, mode):
     """The batched (ensemble) step kernel's template signature."""
     if variant == 0:
         return f"ens_tile_kernel<{p}, {mode}>"
+    if variant == 9:
+        return f"ens_mfma_kernel<{p}, {mode}>"
     ex, ey, w = REG_SCHEDULES[variant - 1]
     return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}>"
 
 
 def kernel_name(variant, p, mode):
-    what = ("LDS-staged tile" if variant == 0 else
+    what = ("LDS-staged tile" if variant == 0 else "matrix-core contractions" if variant == 9 else
             "thread per element, %dx%d element threads" % REG_SCHEDULES[variant - 1][:2])
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
 
@@ -247,6 +251,47 @@ def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
             "speedup_vs_batched_solver": per_solve_batched / per_answer}
 
 
+def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0):
+    """Every element-kernel variant built for (p, precision) on the benchmark's
+    material model at mx x mx elements: mean kernel time over `steps` steps
+    (HIP events), node-updates/s, and with full_steps > 0 the receiver error
+    of each variant after full_steps steps against the FP64 run."""
+    sc = SC.config2(nt=max(steps, full_steps, 1), mx=mx, p=p) if p != 8 or mx != 2048 else SC.config5(nt=max(steps, full_steps, 1))
+    hip, _ = S.libraries()
+    out = {"mx": mx, "p": p, "nodes": sc.nnodes, "steps": steps, "variants": []}
+    ref = None
+    if full_steps > 0:
+        b = SC.build(sc, precision=S.FP64)
+        t0 = time.perf_counter()
+        _, ref = b.solver.advance(full_steps, b.dt)
+        out["fp64_s"] = time.perf_counter() - t0
+        b.solver.close()
+    for name in precisions:
+        prec = S.PRECISIONS[name]
+        for v in range(hip.se2w_kernel_count()):
+            b = SC.build(sc, precision=prec)
+            try:
+                b.solver.ctx.set_kernel(v)
+            except S.Se2wError:
+                b.solver.close()
+                continue
+            b.solver.ctx.set_profiling(True)
+            b.solver.advance(steps, b.dt)
+            _, kern = b.solver.ctx.timing()
+            b.solver.ctx.set_profiling(False)
+            row = {"precision": name, "variant": v, "kernel": kernel_signature(v, p, prec),
+                   "kernel_ms": float(kern[0]), "node_updates_per_s": sc.nnodes / (kern[0] * 1e-3)}
+            if ref is not None:
+                b.solver.close()
+                b = SC.build(sc, precision=prec)
+                b.solver.ctx.set_kernel(v)
+                _, tr = b.solver.advance(full_steps, b.dt)
+                row["receiver_rel_l2_vs_fp64"] = field_errors(tr, ref)
+            out["variants"].append(row)
+            b.solver.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -261,6 +306,11 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
+    ap.add_argument("--variants", nargs=3, type=int, metavar=("MX", "P", "STEPS"),
+                    help="only time every element-kernel variant at MX x MX elements of degree P for STEPS steps "
+                         "(2048 8 STEPS: the layered 2048^2 p=8 workload) and print that as JSON")
+    ap.add_argument("--variants-check", type=int, default=0,
+                    help="with --variants: receiver error of each variant after this many steps against FP64")
     ap.add_argument("--ensemble-steps", type=int, default=1500)
     ap.add_argument("--precisions", default="fp32k,fp32,fp64")
     ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
@@ -272,6 +322,11 @@ def main():
         sys.exit(2)
     import se2dr_fork_amd as S
     from se2dr_fork_amd import scenarios as SC
+
+    if args.variants:
+        mx, p, steps = args.variants
+        print(json.dumps(kernel_variant_bench(S, SC, mx, p, steps, ("fp32", "fp32k"), args.variants_check)))
+        return
 
     sc = SC.config2(nt=max(args.full_steps, 1), mx=args.mx, p=args.p)
     precs = [S.PRECISIONS[x] for x in args.precisions.split(",")]

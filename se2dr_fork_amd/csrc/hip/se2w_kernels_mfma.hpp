@@ -54,13 +54,48 @@ struct MfmaLayout {
   static constexpr int PTS = QX * QY;
 };
 
+// The products of one phase: jobs wave, wave + 4, ... (4 waves), taken G at
+// a time with independent accumulators so that the LDS loads of the B
+// operands and the matrix-core latencies of the G products overlap.
+// fetch(job, s) is this lane's B operand of k-step s; store(job, acc) writes
+// this lane's four results.  `job < NJOB` is uniform across a wave.
+template <int NJOB, int G, typename Fetch, typename Store>
+__device__ __forceinline__ void mfma_phase(int wave, const float (&A)[3], Fetch fetch, Store store) {
+  for (int base = wave; base < NJOB; base += 4 * G) {
+    float b[G][3];
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int job = base + 4 * u;
+#pragma unroll
+      for (int s = 0; s < 3; s++) b[u][s] = job < NJOB ? fetch(job, s) : 0.f;
+    }
+    mfma_f4 acc[G];
+#pragma unroll
+    for (int u = 0; u < G; u++) acc[u] = mfma_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+#pragma unroll
+      for (int u = 0; u < G; u++) acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(A[s], b[u][s], acc[u], 0, 0, 0);
+    }
+#pragma unroll
+    for (int u = 0; u < G; u++) {
+      const int job = base + 4 * u;
+      if (job < NJOB) store(job, acc[u]);
+    }
+  }
+}
+
 template <int P, int MODE>
 __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile) {
   static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
   using Lay = MfmaLayout<P>;
   constexpr int NB = Lay::NB, TEX = Lay::TEX, TEY = Lay::TEY;
   constexpr int NXL = Lay::NXL, NYL = Lay::NYL, QX = Lay::QX, QY = Lay::QY;
+  constexpr int NE = (TEX + 1) * (TEY + 1);
+  static_assert(NE <= kThreads, "one thread per tile element");
   __shared__ float su[2][NYL][NXL];
+  __shared__ float econ[5][NE];        // 1/J00, 1/J11, J00 J11, lambda, mu (0 outside the mesh)
+  __shared__ float sW[NB * NB];
   __shared__ float S[Lay::SS];         // [3][QY][QX]: sxx, syy, sxy times detJ w_q
   __shared__ float work[Lay::WORK];
   float *Dx = work;                    // [2][NYL][QX]
@@ -82,7 +117,8 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
     Ad[s] = (lm < NB && k < NB) ? prm.D[lm * NB + k] : 0.f;
     At[s] = (lm < NB && k < NB) ? prm.D[k * NB + lm] : 0.f;
   }
-  // Phase A: displacement of the tile's nodes (+ halo) into LDS.
+  // Phase A: displacement of the tile's nodes (+ halo) into LDS; per-element
+  // constants (inverse Jacobians, determinant, Lame parameters).
   for (int k = tid; k < NXL * NYL; k += kThreads) {
     const int ly = k / NXL, lx = k - ly * NXL;
     const int gx = gx0 + lx, gy = gy0 + ly;
@@ -91,107 +127,113 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
     su[0][ly][lx] = uv.x;
     su[1][ly][lx] = uv.y;
   }
+  if (tid < NE) {
+    const int el = tid % (TEX + 1), er = tid / (TEX + 1);
+    const int ex = ex0 - 1 + el, ey = ey0 - 1 + er;
+    float c0 = 0.f, c1 = 0.f, c2 = 0.f, c3 = 0.f, c4 = 0.f;
+    if (ex >= 0 && ex < prm.mx && ey >= 0 && ey < prm.my) {
+      const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
+      const int e = ex + ey * prm.mx;
+      c0 = 1.f / J00;
+      c1 = 1.f / J11;
+      c2 = J00 * J11;
+      c3 = prm.lambda[e];
+      c4 = prm.mu[e];
+    }
+    econ[0][tid] = c0;
+    econ[1][tid] = c1;
+    econ[2][tid] = c2;
+    econ[3][tid] = c3;
+    econ[4][tid] = c4;
+  }
+  for (int k = tid; k < NB * NB; k += kThreads) sW[k] = prm.w2[k];
   __syncthreads();
   // B1: xi-derivatives along every node row, per element column.
-  for (int job = wave; job < Lay::JB1; job += 4) {
-    const int c = job & 1, rest = job >> 1;
-    const int el = rest % (TEX + 1), row0 = 16 * (rest / (TEX + 1));
-    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int row = row0 + lm;
+  mfma_phase<Lay::JB1, 4>(
+      wave, Ad,
+      [&](int job, int s) {
+        const int c = job & 1, rest = job >> 1;
+        const int el = rest % (TEX + 1), row = 16 * (rest / (TEX + 1)) + lm, k = 4 * s + lk;
+        return (k < NB && row < NYL) ? su[c][row][el * P + k] : 0.f;
+      },
+      [&](int job, const mfma_f4 &acc) {
+        const int c = job & 1, rest = job >> 1;
+        const int el = rest % (TEX + 1), row = 16 * (rest / (TEX + 1)) + lm;
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
-      const int k = 4 * s + lk;
-      const float b = (k < NB && row < NYL) ? su[c][row][el * P + k] : 0.f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ad[s], b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int qi = 4 * lk + r;
-      if (qi < NB && row < NYL) Dx[(c * NYL + row) * QX + el * NB + qi] = acc[r];
-    }
-  }
+        for (int r = 0; r < 4; r++) {
+          const int qi = 4 * lk + r;
+          if (qi < NB && row < NYL) Dx[(c * NYL + row) * QX + el * NB + qi] = acc[r];
+        }
+      });
   // B2: eta-derivatives along every node column, per element row.
-  for (int job = wave; job < Lay::JB2; job += 4) {
-    const int c = job & 1, rest = job >> 1;
-    const int er = rest % (TEY + 1), col0 = 16 * (rest / (TEY + 1));
-    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
-    const int col = col0 + lm;
+  mfma_phase<Lay::JB2, 4>(
+      wave, Ad,
+      [&](int job, int s) {
+        const int c = job & 1, rest = job >> 1;
+        const int er = rest % (TEY + 1), col = 16 * (rest / (TEY + 1)) + lm, k = 4 * s + lk;
+        return (k < NB && col < NXL) ? su[c][er * P + k][col] : 0.f;
+      },
+      [&](int job, const mfma_f4 &acc) {
+        const int c = job & 1, rest = job >> 1;
+        const int er = rest % (TEY + 1), col = 16 * (rest / (TEY + 1)) + lm;
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
-      const int k = 4 * s + lk;
-      const float b = (k < NB && col < NXL) ? su[c][er * P + k][col] : 0.f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(Ad[s], b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int qj = 4 * lk + r;
-      if (qj < NB && col < NXL) Dy[(c * QY + er * NB + qj) * NXL + col] = acc[r];
-    }
-  }
+        for (int r = 0; r < 4; r++) {
+          const int qj = 4 * lk + r;
+          if (qj < NB && col < NXL) Dy[(c * QY + er * NB + qj) * NXL + col] = acc[r];
+        }
+      });
   __syncthreads();
   // B3: weighted stresses at the quadrature points.
   for (int pnt = tid; pnt < Lay::PTS; pnt += kThreads) {
     const int qrow = pnt / QX, qcol = pnt - qrow * QX;
     const int er = qrow / NB, qj = qrow - er * NB, el = qcol / NB, qi = qcol - el * NB;
-    const int ex = ex0 - 1 + el, ey = ey0 - 1 + er;
-    float sxx = 0.f, syy = 0.f, sxy = 0.f;
-    if (ex >= 0 && ex < prm.mx && ey >= 0 && ey < prm.my) {
-      const int row = er * P + qj, col = el * P + qi;
-      const float dxux = Dx[(0 * NYL + row) * QX + qcol], dxuy = Dx[(1 * NYL + row) * QX + qcol];
-      const float dyux = Dy[(0 * QY + qrow) * NXL + col], dyuy = Dy[(1 * QY + qrow) * NXL + col];
-      const int e = ex + ey * prm.mx;
-      const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
-      const float iJ00 = 1.f / J00, iJ11 = 1.f / J11;
-      const float fac = (J00 * J11) * prm.w2[qi + qj * NB];
-      const float lam = prm.lambda[e], mu = prm.mu[e], c11 = 2.f * mu + lam;
-      const float e0 = iJ00 * dxux, e1 = iJ11 * dyuy, e2 = iJ11 * dyux + iJ00 * dxuy;
-      sxx = fac * (c11 * e0 + lam * e1);
-      syy = fac * (lam * e0 + c11 * e1);
-      sxy = fac * (mu * e2);
-    }
-    S[0 * QY * QX + pnt] = sxx;
-    S[1 * QY * QX + pnt] = syy;
-    S[2 * QY * QX + pnt] = sxy;
+    const int row = er * P + qj, col = el * P + qi, ei = er * (TEX + 1) + el;
+    const float dxux = Dx[(0 * NYL + row) * QX + qcol], dxuy = Dx[(1 * NYL + row) * QX + qcol];
+    const float dyux = Dy[(0 * QY + qrow) * NXL + col], dyuy = Dy[(1 * QY + qrow) * NXL + col];
+    const float iJ00 = econ[0][ei], iJ11 = econ[1][ei], lam = econ[3][ei], mu = econ[4][ei];
+    const float fac = econ[2][ei] * sW[qi + qj * NB], c11 = 2.f * mu + lam;  // zero outside the mesh
+    const float e0 = iJ00 * dxux, e1 = iJ11 * dyuy, e2 = iJ11 * dyux + iJ00 * dxuy;
+    S[0 * QY * QX + pnt] = fac * (c11 * e0 + lam * e1);
+    S[1 * QY * QX + pnt] = fac * (lam * e0 + c11 * e1);
+    S[2 * QY * QX + pnt] = fac * (mu * e2);
   }
   __syncthreads();  // S complete; the derivatives are dead, R and C reuse their space
   // C1: row terms R(c, el, i, qrow); c = 0 from sxx (x force), c = 1 from sxy (y force)
-  for (int job = wave; job < Lay::JC1; job += 4) {
-    const int c = job & 1, rest = job >> 1;
-    const int el = rest % (TEX + 1), q0 = 16 * (rest / (TEX + 1));
-    const float *Src = S + (c == 0 ? 0 : 2) * QY * QX;
-    const int qrow = q0 + lm;
-    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  mfma_phase<Lay::JC1, 4>(
+      wave, At,
+      [&](int job, int s) {
+        const int c = job & 1, rest = job >> 1;
+        const int el = rest % (TEX + 1), qrow = 16 * (rest / (TEX + 1)) + lm, k = 4 * s + lk;
+        const float *Src = S + (c == 0 ? 0 : 2) * QY * QX;
+        return (k < NB && qrow < QY) ? Src[qrow * QX + el * NB + k] : 0.f;
+      },
+      [&](int job, const mfma_f4 &acc) {
+        const int c = job & 1, rest = job >> 1;
+        const int el = rest % (TEX + 1), qrow = 16 * (rest / (TEX + 1)) + lm;
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
-      const int k = 4 * s + lk;
-      const float b = (k < NB && qrow < QY) ? Src[qrow * QX + el * NB + k] : 0.f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(At[s], b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int i = 4 * lk + r;
-      if (i < NB && qrow < QY) Rr[(c * QX + el * NB + i) * QY + qrow] = acc[r];
-    }
-  }
+        for (int r = 0; r < 4; r++) {
+          const int i = 4 * lk + r;
+          if (i < NB && qrow < QY) Rr[(c * QX + el * NB + i) * QY + qrow] = acc[r];
+        }
+      });
   // C2: column terms C(c, er, j, qcol); c = 0 from sxy (x force), c = 1 from syy (y force)
-  for (int job = wave; job < Lay::JC2; job += 4) {
-    const int c = job & 1, rest = job >> 1;
-    const int er = rest % (TEY + 1), q0 = 16 * (rest / (TEY + 1));
-    const float *Src = S + (c == 0 ? 2 : 1) * QY * QX;
-    const int qcol = q0 + lm;
-    mfma_f4 acc = {0.f, 0.f, 0.f, 0.f};
+  mfma_phase<Lay::JC2, 4>(
+      wave, At,
+      [&](int job, int s) {
+        const int c = job & 1, rest = job >> 1;
+        const int er = rest % (TEY + 1), qcol = 16 * (rest / (TEY + 1)) + lm, k = 4 * s + lk;
+        const float *Src = S + (c == 0 ? 2 : 1) * QY * QX;
+        return (k < NB && qcol < QX) ? Src[(er * NB + k) * QX + qcol] : 0.f;
+      },
+      [&](int job, const mfma_f4 &acc) {
+        const int c = job & 1, rest = job >> 1;
+        const int er = rest % (TEY + 1), qcol = 16 * (rest / (TEY + 1)) + lm;
 #pragma unroll
-    for (int s = 0; s < 3; s++) {
-      const int k = 4 * s + lk;
-      const float b = (k < NB && qcol < QX) ? Src[(er * NB + k) * QX + qcol] : 0.f;
-      acc = __builtin_amdgcn_mfma_f32_16x16x4f32(At[s], b, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; r++) {
-      const int j = 4 * lk + r;
-      if (j < NB && qcol < QX) Cc[(c * QY + er * NB + j) * QX + qcol] = acc[r];
-    }
-  }
+        for (int r = 0; r < 4; r++) {
+          const int j = 4 * lk + r;
+          if (j < NB && qcol < QX) Cc[(c * QY + er * NB + j) * QX + qcol] = acc[r];
+        }
+      });
   __syncthreads();
   // D: owned nodes: element contributions in ascending element order, update.
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
@@ -219,7 +261,8 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
         const int ex = exs[a], ey = eys[b];
         const int el = ex - ex0 + 1, er = ey - ey0 + 1;
         const int i = gx - ex * P, j = gy - ey * P;
-        const float iJ00 = 1.f / (0.5f * prm.elem_dx[ex]), iJ11 = 1.f / (0.5f * prm.elem_dy[ey]);
+        const int ei = er * (TEX + 1) + el;
+        const float iJ00 = econ[0][ei], iJ11 = econ[1][ei];
         const int qr = er * NB + j, qc = el * NB + i;
         fx += -(iJ00 * Rr[(0 * QX + qc) * QY + qr] + iJ11 * Cc[(0 * QY + qr) * QX + qc]);
         fy += -(iJ00 * Rr[(1 * QX + qc) * QY + qr] + iJ11 * Cc[(1 * QY + qr) * QX + qc]);
