@@ -259,12 +259,14 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
     sc = SC.config2(nt=max(steps, full_steps, 1), mx=mx, p=p) if p != 8 or mx != 2048 else SC.config5(nt=max(steps, full_steps, 1))
     hip, _ = S.libraries()
     out = {"mx": mx, "p": p, "nodes": sc.nnodes, "steps": steps, "variants": []}
-    ref = None
+    ref = ref_state = None
     if full_steps > 0:
         b = SC.build(sc, precision=S.FP64)
         t0 = time.perf_counter()
         _, ref = b.solver.advance(full_steps, b.dt)
         out["fp64_s"] = time.perf_counter() - t0
+        ref_state = b.solver.ctx.get_state()[:2]
+        out["simulated_s"] = full_steps * b.dt
         b.solver.close()
     for name in precisions:
         prec = S.PRECISIONS[name]
@@ -287,6 +289,9 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
                 b.solver.ctx.set_kernel(v)
                 _, tr = b.solver.advance(full_steps, b.dt)
                 row["receiver_rel_l2_vs_fp64"] = field_errors(tr, ref)
+                u, v, _ = b.solver.ctx.get_state()
+                row["state_rel_l2_vs_fp64"] = {"u": rel_l2(u, ref_state[0]), "v": rel_l2(v, ref_state[1])}
+                del u, v
             out["variants"].append(row)
             b.solver.close()
     return out
