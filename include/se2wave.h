@@ -61,6 +61,12 @@ int se2w_spec_set_constant_material_velocity(SE2WSpec spec, double vp, double vs
 /* Cell-wise material arrays (element order e = ei + ej*mx). */
 int se2w_spec_set_cell_material(SE2WSpec spec, const double *lambda, const double *mu, const double *rho);
 int se2w_spec_set_cell_material_velocity(SE2WSpec spec, const double *vp, const double *vs, const double *rho);
+/* SpecFECtxSetPerturbedMaterialProperties_Velocity (se2wave.c:1177-1236):
+ * per cell Vp ~ U[vp0 - dvp, vp0 + dvp), Vs ~ U[vs0 - dvs, ...), rho ~
+ * U[rho0 - drho, ...), drawn in element order from the drand48 stream seeded
+ * 1, 2 and 3 respectively; prints the reference's three [material] lines. */
+int se2w_spec_set_perturbed_material_velocity(SE2WSpec spec, double vp0, double delta_vp, double vs0, double delta_vs,
+                                              double rho0, double delta_rho);
 
 typedef struct {
   int32_t mx, my, p, nx, ny, npe;
@@ -153,6 +159,9 @@ uint64_t se2w_rng_next(uint64_t *state);
 double se2w_rng_uniform(uint64_t *state);
 /* Per cell, element order: Vp ~ U[vp_lo,vp_hi], Vs = Vp/sqrt(3),
  * rho ~ U[rho_lo,rho_hi] (draws vp then rho per cell). */
+/* POSIX drand48 generator (the reference's PETSCRAND48 random type). */
+void se2w_rand48_seed(uint64_t *state, long seed);
+double se2w_rand48_next(uint64_t *state);
 int se2w_material_random_cells(int mx, int my, uint64_t seed, double vp_lo, double vp_hi, double rho_lo,
                                double rho_hi, double *vp, double *vs, double *rho);
 /* nlayers horizontal layers of equal height; one (Vp, rho) draw per layer. */

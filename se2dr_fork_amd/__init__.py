@@ -119,6 +119,10 @@ def _declare(hip, host):
     host.se2w_spec_set_constant_material_velocity.argtypes = [v, d, d, d]
     host.se2w_spec_set_cell_material.argtypes = [v, _dp, _dp, _dp]
     host.se2w_spec_set_cell_material_velocity.argtypes = [v, _dp, _dp, _dp]
+    host.se2w_spec_set_perturbed_material_velocity.argtypes = [v, d, d, d, d, d, d]
+    host.se2w_rand48_seed.argtypes = [C.POINTER(C.c_uint64), C.c_long]
+    host.se2w_rand48_next.argtypes = [C.POINTER(C.c_uint64)]
+    host.se2w_rand48_next.restype = d
     host.se2w_spec_get_info.argtypes = [v, v]
     host.se2w_spec_get_coords.argtypes = [v, _dp]
     host.se2w_spec_get_material.argtypes = [v, _dp, _dp, _dp]
@@ -238,6 +242,10 @@ class SpecFE:
     def set_cell_material_velocity(self, vp, vs, rho):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (vp, vs, rho)]
         _hchk(self._host.se2w_spec_set_cell_material_velocity(self.h, *[_ptr(x) for x in a]))
+
+    def set_perturbed_material_velocity(self, vp0, dvp, vs0, dvs, rho0, drho):
+        """The reference's RAND48-perturbed per-cell model (se2wave.c:1177-1236)."""
+        _hchk(self._host.se2w_spec_set_perturbed_material_velocity(self.h, vp0, dvp, vs0, dvs, rho0, drho))
 
     def set_cell_material(self, lam, mu, rho):
         a = [np.ascontiguousarray(x, dtype=np.float64) for x in (lam, mu, rho)]

@@ -11,6 +11,7 @@
  *   -precision fp64|fp32|fp32k, -schedule fused|split
  *   -scale sx,sy -shift tx,ty  domain = [0,1]^2 * scale + shift (demo: 2000,-1000)
  *   -material_vp/-material_vs/-material_rho   constant material
+ *   -material_perturbed vp0,dvp,vs0,dvs,rho0,drho    the reference's RAND48 perturbed model
  *   -material_random seed,vp_lo,vp_hi,rho_lo,rho_hi   per-cell, Vs = Vp/sqrt(3)
  *   -material_layers n,seed,vp_lo,vp_hi,rho_lo,rho_hi horizontal layers
  *   -src_xy x0,y0,...  -src_type moment|force  -src_values v0..v3 per source
@@ -134,6 +135,12 @@ static int setup_material(SE2WOptions opts, SE2WSpec spec, int mx, int my)
 {
   double arr[8];
   int n = 8, found;
+  CHK(se2w_options_get_real_array(opts, "-material_perturbed", arr, &n, &found));
+  if (found) {
+    if (n != 6) { fprintf(stderr, "[se2wave] -material_perturbed vp0,dvp,vs0,dvs,rho0,drho\n"); return SE2W_HOST_ERR_ARG; }
+    return se2w_spec_set_perturbed_material_velocity(spec, arr[0], arr[1], arr[2], arr[3], arr[4], arr[5]);
+  }
+  n = 8;
   CHK(se2w_options_get_real_array(opts, "-material_random", arr, &n, &found));
   if (found) {
     double *vp = malloc(sizeof(double) * mx * my), *vs = malloc(sizeof(double) * mx * my), *rho = malloc(sizeof(double) * mx * my);

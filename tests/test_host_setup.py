@@ -136,3 +136,47 @@ def test_host_sources_evaluate_matches_gpu_convention():
             ref[2 * n] += psi * vals[2 * k]
             ref[2 * n + 1] += psi * vals[2 * k + 1]
     np.testing.assert_array_equal(g, ref)
+
+
+def test_rand48_matches_the_c_library():
+    """The perturbed model's generator is POSIX drand48 (PETSc's PETSCRAND48
+    type calls srand48/drand48): the restatement must give libc's bits."""
+    import ctypes
+    import ctypes.util
+
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    _, host = S.libraries()
+    for seed in [1, 2, 3, 12345, -7]:
+        libc.srand48(seed)
+        st = ctypes.c_uint64()
+        host.se2w_rand48_seed(ctypes.byref(st), seed)
+        for _ in range(1000):
+            assert host.se2w_rand48_next(ctypes.byref(st)) == libc.drand48()
+
+
+def test_perturbed_material_follows_the_reference_draws():
+    """SpecFECtxSetPerturbedMaterialProperties_Velocity (se2wave.c:1177-1236):
+    VecSetRandom with interval [x0 - d, x0 + d] and seeds 1, 2, 3."""
+    import ctypes
+    import ctypes.util
+
+    libc = ctypes.CDLL(ctypes.util.find_library("c"))
+    libc.drand48.restype = ctypes.c_double
+    libc.srand48.argtypes = [ctypes.c_long]
+    mx, my = 7, 5
+
+    def draws(seed, x0, d):
+        libc.srand48(seed)
+        lo, hi = x0 - d, x0 + d
+        return np.array([(hi - lo) * libc.drand48() + lo for _ in range(mx * my)])
+
+    spec = S.SpecFE(mx, my, 3)
+    spec.scale_coords((100.0, 100.0), (0.0, 0.0))
+    spec.set_perturbed_material_velocity(5000.0, 250.0, 2900.0, 100.0, 2600.0, 50.0)
+    vp, vs, rho = draws(1, 5000.0, 250.0), draws(2, 2900.0, 100.0), draws(3, 2600.0, 50.0)
+    lam, mu, r = spec.material()
+    np.testing.assert_array_equal(r, rho)
+    np.testing.assert_array_equal(mu, vs * vs * rho)
+    np.testing.assert_array_equal(lam, vp * vp * rho - 2.0 * (vs * vs * rho))
