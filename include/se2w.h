@@ -82,6 +82,15 @@ int se2w_set_receivers(se2w_ctx *ctx, int32_t nr, const int32_t *elem, const int
 
 /* Sets / reads the state vectors u, v, a (each 2*nx*ny doubles; NULL skips).
  * Replaces VecGetArray access to the u, v, a Vecs of se2wave_demo. */
+/* Replaces the sampling of RecordUV and RecordUV_interp (se2wave.c:2351-2581):
+ * each step records u and v at every node of element elem[r] (element order
+ * e = ei + ej*mx).  se2w_get_element_receivers returns, for every step k of
+ * the last run, trace[k][r][4] = sum over the element's nodes q (local index
+ * q = i + j*(p+1), ascending) of weights[r][q] * (ux, uy, vx, vy) at node q:
+ * Lagrange basis values interpolate at a point, a unit weight picks the
+ * nearest node. */
+int se2w_set_element_receivers(se2w_ctx *ctx, int32_t n, const int32_t *elem);
+int se2w_get_element_receivers(se2w_ctx *ctx, const double *weights, double *trace);
 int se2w_set_state(se2w_ctx *ctx, const double *u, const double *v, const double *a);
 int se2w_get_state(se2w_ctx *ctx, double *u, double *v, double *a);
 

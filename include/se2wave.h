@@ -138,6 +138,24 @@ int se2w_receivers_write_header(SE2WReceivers recv, const char *dir);
  * (2908-2962). */
 int se2w_receivers_append_rows(SE2WReceivers recv, int nsteps, const double *times, const double *trace);
 
+/* Single-station receivers of displacement and velocity: RecordUV
+ * (interpolate = 0: the nearest node of the containing element,
+ * se2wave.c:2351-2443) and RecordUV_interp (interpolate = 1: the element's
+ * Lagrange interpolant at the station, se2wave.c:2445-2581).  The containing
+ * element is found by truncation without clamping, as there; a station with
+ * none is rejected.  source_implementation (-1..3) names the file as the
+ * reference does ("<impl>source-receiverCP-..." / "<impl>source-receiver-...").
+ * se2w_receiver_uv_get returns the element, the npe weights of its nodes and
+ * the mapped (nearest-node or interpolated) coordinate. */
+typedef struct se2w_receiver_uv *SE2WReceiverUV;
+int se2w_receiver_uv_create(SE2WSpec spec, const double xr[2], int interpolate, int source_implementation,
+                            SE2WReceiverUV *receiver);
+int se2w_receiver_uv_destroy(SE2WReceiverUV *receiver);
+int se2w_receiver_uv_get(SE2WReceiverUV receiver, int32_t *elem, double *weights, double mapped[2]);
+int se2w_receiver_uv_write_header(SE2WReceiverUV receiver, const char *dir);
+/* rows[nsteps][4] = ux uy vx vy */
+int se2w_receiver_uv_append_rows(SE2WReceiverUV receiver, int nsteps, const double *times, const double *rows);
+
 /* ---------------------------------------------------------------------- */
 /* Options database (PetscOptionsGet{Int,Real,Bool,RealArray} analogue)     */
 /* ---------------------------------------------------------------------- */
@@ -186,6 +204,11 @@ double se2w_solver_time(SE2WSolver solver);
 int se2w_solver_steps_done(SE2WSolver solver);
 /* times[nsteps] and trace[nsteps][nr][7] of the last advance. */
 int se2w_solver_get_trace(SE2WSolver solver, double *times, double *trace);
+/* Single-station receivers sampled by the solver from the next advance on;
+ * the objects must outlive the solver.  se2w_solver_get_uv_trace returns
+ * trace[nsteps][n][4] (ux uy vx vy) of the last advance. */
+int se2w_solver_set_uv_receivers(SE2WSolver solver, int n, SE2WReceiverUV *receivers);
+int se2w_solver_get_uv_trace(SE2WSolver solver, double *trace);
 /* Advances n solvers together, one batched GPU launch per step for all of
  * them; dt[i] is solver i's step.  ens is an ensemble of the solvers'
  * contexts (se2w_solver_context) in the same order.  Each solver's time, step

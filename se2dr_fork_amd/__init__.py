@@ -158,6 +158,15 @@ def _declare(hip, host):
     host.se2w_solver_time.restype = d
     host.se2w_solver_get_trace.argtypes = [v, _dp, _dp]
     host.se2w_solvers_advance.argtypes = [C.c_int, C.POINTER(v), v, C.c_int, _dp]
+    host.se2w_receiver_uv_create.argtypes = [v, _dp, C.c_int, C.c_int, C.POINTER(v)]
+    host.se2w_receiver_uv_destroy.argtypes = [C.POINTER(v)]
+    host.se2w_receiver_uv_get.argtypes = [v, _ip, _dp, _dp]
+    host.se2w_receiver_uv_write_header.argtypes = [v, C.c_char_p]
+    host.se2w_receiver_uv_append_rows.argtypes = [v, C.c_int, _dp, _dp]
+    host.se2w_solver_set_uv_receivers.argtypes = [v, C.c_int, C.POINTER(v)]
+    host.se2w_solver_get_uv_trace.argtypes = [v, _dp]
+    hip.se2w_set_element_receivers.argtypes = [v, i32, _ip]
+    hip.se2w_get_element_receivers.argtypes = [v, _dp, _dp]
     host.se2w_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
 
 
@@ -382,6 +391,46 @@ class Receivers:
             pass
 
 
+class ReceiverUV:
+    """Single-station receiver: RecordUV (interpolate=False, nearest node of
+    the containing element) or RecordUV_interp (interpolate=True, Lagrange
+    interpolation), se2wave.c:2351-2581."""
+
+    def __init__(self, spec: SpecFE, xr, interpolate=False, source_implementation=1):
+        _, self._host = libraries()
+        self.spec = spec
+        xy = np.ascontiguousarray(xr, dtype=np.float64)
+        self.h = C.c_void_p()
+        _hchk(self._host.se2w_receiver_uv_create(spec.h, _ptr(xy), 1 if interpolate else 0, source_implementation,
+                                                 C.byref(self.h)))
+
+    def get(self):
+        """(element, weights[npe], mapped coordinate)."""
+        e = C.c_int32()
+        w = np.zeros((self.spec.p + 1) ** 2)
+        m = np.zeros(2)
+        _hchk(self._host.se2w_receiver_uv_get(self.h, C.byref(e), _ptr(w), _ptr(m)))
+        return e.value, w, m
+
+    def write_header(self, directory: str):
+        _hchk(self._host.se2w_receiver_uv_write_header(self.h, directory.encode()))
+
+    def append_rows(self, times, rows):
+        t = np.ascontiguousarray(times, dtype=np.float64)
+        r = np.ascontiguousarray(rows, dtype=np.float64)
+        _hchk(self._host.se2w_receiver_uv_append_rows(self.h, len(t), _ptr(t), _ptr(r)))
+
+    def close(self):
+        if self.h:
+            self._host.se2w_receiver_uv_destroy(C.byref(self.h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class Solver:
     """se2wave_demo's time loop on the GPU (host-evaluated STFs, accumulated time)."""
 
@@ -404,6 +453,18 @@ class Solver:
         trace = np.zeros(nsteps * max(self.nr, 1) * 7)
         _hchk(self._host.se2w_solver_get_trace(self.h, _ptr(times), _ptr(trace)))
         return times, trace[: nsteps * self.nr * 7].reshape(nsteps, self.nr, 7)
+
+    def set_uv_receivers(self, receivers: Sequence["ReceiverUV"]):
+        self.uv_receivers = list(receivers)
+        arr = (C.c_void_p * max(len(self.uv_receivers), 1))(*[r.h for r in self.uv_receivers])
+        _hchk(self._host.se2w_solver_set_uv_receivers(self.h, len(self.uv_receivers), arr))
+
+    def uv_trace(self, nsteps: int) -> np.ndarray:
+        """[nsteps][n][4] (ux uy vx vy) of the last advance of nsteps steps."""
+        n = len(getattr(self, "uv_receivers", []))
+        out = np.zeros(max(nsteps * n * 4, 1))
+        _hchk(self._host.se2w_solver_get_uv_trace(self.h, _ptr(out)))
+        return out[: nsteps * n * 4].reshape(nsteps, n, 4)
 
     @property
     def time(self) -> float:

@@ -147,6 +147,13 @@ struct se2w_ctx {
   size_t rec_cap = 0;
   void *h_stage = nullptr;  // pinned host staging buffer for the record download
   size_t stage_cap = 0;
+  // element receivers (u, v at every node of an element, for interpolation)
+  int ner = 0;
+  std::vector<int> h_erec_elem;
+  int *d_erec_elem = nullptr, *d_terec_ptr = nullptr, *d_terec_idx = nullptr;
+  void *d_erec = nullptr;
+  size_t erec_cap = 0;
+  int erec_steps = 0;
   int rec_steps = 0;
   std::vector<double> trace;
   // timing
@@ -295,6 +302,8 @@ StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double d
   p.tile_rec_ptr = c->d_trec_ptr; p.tile_rec_idx = c->d_trec_idx; p.rec_elem = c->d_rec_elem;
   p.rec_gll = c->d_rec_gll;
   p.rec = (R *)c->d_rec; p.nr = c->nr;
+  p.tile_erec_ptr = c->d_terec_ptr; p.tile_erec_idx = c->d_terec_idx; p.erec_elem = c->d_erec_elem;
+  p.erec = (R *)c->d_erec; p.ner = c->ner;
   p.step = step;
   // dt, 0.5*dt and 0.5*dt*dt are formed in FP64 as in se2wave.c:4726-4745
   p.dt = (R)dt;
@@ -559,10 +568,10 @@ int se2w_destroy(se2w_ctx *c) {
   if (!c) return SE2W_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   void **bufs[] = {&c->U[0], &c->U[1], &c->C[0], &c->C[1], &c->V, &c->A, &c->F, &c->Utmp, &c->mass, &c->lambda,
-                   &c->mu, &c->edx, &c->edy, &c->D, &c->w2, &c->d_src_vals, &c->d_psi, &c->d_rec};
+                   &c->mu, &c->edx, &c->edy, &c->D, &c->w2, &c->d_src_vals, &c->d_psi, &c->d_rec, &c->d_erec};
   for (void **b : bufs) dev_free(*b);
   int **ibufs[] = {&c->d_src_elem, &c->d_tsrc_ptr, &c->d_tsrc_idx, &c->d_rec_elem, &c->d_rec_gll,
-                    &c->d_trec_ptr, &c->d_trec_idx};
+                    &c->d_trec_ptr, &c->d_trec_idx, &c->d_erec_elem, &c->d_terec_ptr, &c->d_terec_idx};
   for (int **b : ibufs) {
     void *p = *b;
     dev_free(p);
@@ -680,6 +689,55 @@ int se2w_set_receivers(se2w_ctx *c, int32_t nr, const int32_t *elem, const int32
   return SE2W_OK;
 }
 
+int se2w_set_element_receivers(se2w_ctx *c, int32_t n, const int32_t *elem) {
+  if (!c || n < 0 || (n > 0 && !elem)) return fail(SE2W_ERR_ARG, "bad element-receiver arguments");
+  HIPCHK(hipSetDevice(c->device));
+  const int ne = c->mx * c->my;
+  for (int r = 0; r < n; r++)
+    if (elem[r] < 0 || elem[r] >= ne) return fail(SE2W_ERR_ARG, "element receiver %d: element %d out of range", r, elem[r]);
+  c->h_erec_elem.assign(elem, elem + n);
+  std::vector<int> ptr, idx;
+  build_tile_lists(c, c->h_erec_elem, ptr, idx);
+  int ierr;
+  if ((ierr = upload<int>((void **)&c->d_erec_elem, c->h_erec_elem.data(), n, c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_terec_ptr, ptr.data(), ptr.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_terec_idx, idx.data(), idx.size(), c->stream)))
+    return ierr;
+  c->ner = n;
+  c->erec_steps = 0;
+  return SE2W_OK;
+}
+
+int se2w_get_element_receivers(se2w_ctx *c, const double *weights, double *trace) {
+#pragma clang fp contract(off)
+  if (!c || (c->ner > 0 && (!weights || !trace))) return fail(SE2W_ERR_ARG, "null argument");
+  if (c->ner == 0 || c->erec_steps == 0) return SE2W_OK;
+  HIPCHK(hipSetDevice(c->device));
+  const int npe = c->npe, ner = c->ner;
+  const size_t n = (size_t)c->erec_steps * ner * npe * 4;
+  std::vector<double> rec(n);
+  if (c->mode == SE2W_FP64) {
+    HIPCHK(hipMemcpyAsync(rec.data(), c->d_erec, n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<float> tmp(n);
+    HIPCHK(hipMemcpyAsync(tmp.data(), c->d_erec, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) rec[i] = (double)tmp[i];
+  }
+  // sum over the element's nodes in ascending local index (se2wave.c:2563-2571)
+  for (int k = 0; k < c->erec_steps; k++)
+    for (int r = 0; r < ner; r++) {
+      const double *R = &rec[((size_t)k * ner + r) * npe * 4];
+      const double *w = weights + (size_t)r * npe;
+      double s[4] = {0.0, 0.0, 0.0, 0.0};
+      for (int q = 0; q < npe; q++)
+        for (int f = 0; f < 4; f++) s[f] = s[f] + w[q] * R[q * 4 + f];
+      for (int f = 0; f < 4; f++) trace[((size_t)k * ner + r) * 4 + f] = s[f];
+    }
+  return SE2W_OK;
+}
+
 int se2w_set_state(se2w_ctx *c, const double *u, const double *v, const double *a) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
@@ -776,7 +834,16 @@ int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
       c->rec_cap = n;
     }
   }
+  if (c->ner > 0) {
+    const size_t n = (size_t)nsteps * c->ner * c->npe * 4;
+    if (n > c->erec_cap) {
+      dev_free(c->d_erec);
+      if ((ierr = dev_alloc(&c->d_erec, n * rs))) return ierr;
+      c->erec_cap = n;
+    }
+  }
   c->rec_steps = nsteps;
+  c->erec_steps = nsteps;
   c->trace.clear();
   return SE2W_OK;
 }
@@ -932,6 +999,10 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if ((ierr = upload<int>((void **)&c->d_trec_ptr, ptr.data(), ptr.size(), c->stream)) ||
       (ierr = upload<int>((void **)&c->d_trec_idx, idx.data(), idx.size(), c->stream)))
     return ierr;
+  build_tile_lists(c, c->h_erec_elem, ptr, idx);
+  if ((ierr = upload<int>((void **)&c->d_terec_ptr, ptr.data(), ptr.size(), c->stream)) ||
+      (ierr = upload<int>((void **)&c->d_terec_idx, idx.data(), idx.size(), c->stream)))
+    return ierr;
   return SE2W_OK;
 }
 
@@ -989,6 +1060,7 @@ int snapshot_restore(se2w_ctx *c, Snapshot &sn) {
   c->ns = sn.ns;
   c->nr = sn.nr;
   c->rec_steps = 0;
+  c->erec_steps = 0;
   return SE2W_OK;
 }
 

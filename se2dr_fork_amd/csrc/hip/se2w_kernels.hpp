@@ -84,8 +84,11 @@ template <typename R> struct StepParams {
   int ns;
   // receivers (CSR per tile)
   const int *tile_rec_ptr, *tile_rec_idx, *rec_elem, *rec_gll;
-  R *rec;             // [nsteps_run][nr][npe][6]
+  const int *tile_erec_ptr, *tile_erec_idx, *erec_elem;  // element receivers (all nodes of an element)
+  R *rec;             // [nsteps_run][nr][rec_stride(p)]
   int nr;
+  R *erec;            // [nsteps_run][ner][npe][4]: ux uy vx vy at every node of the element
+  int ner;
   int step;           // step index inside the current run
   R dt, hdt, hdt2;    // dt, 0.5*dt, 0.5*dt*dt (formed in FP64 on the host)
   int last;           // 1: keep the corrected state (no predictor), write a
@@ -252,6 +255,18 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int t
     if (lj == gj && li >= 0 && li <= P) o[6 + NB + li] = vy;
     if (li == gi && lj == gj) {
       o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy; o[4] = ax; o[5] = ay;
+    }
+  }
+  if (prm.ner > 0) {  // element receivers: every node of the element
+    const int q0 = prm.tile_erec_ptr[tile], q1 = prm.tile_erec_ptr[tile + 1];
+    for (int t = q0; t < q1; t++) {
+      const int r = prm.tile_erec_idx[t];
+      const int e = prm.erec_elem[r];
+      const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
+      if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
+        R *o = prm.erec + (((long)prm.step * prm.ner + r) * (NB * NB) + li + lj * NB) * 4;
+        o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy;
+      }
     }
   }
 }

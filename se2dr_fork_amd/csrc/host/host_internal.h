@@ -60,4 +60,16 @@ struct se2w_receivers {
   int header_written;
 };
 
+struct se2w_receiver_uv {
+  struct se2w_spec *spec;
+  double xr[2];
+  int interpolate;            /* 0: nearest node (RecordUV), 1: interpolated (RecordUV_interp) */
+  int source_implementation;  /* -1..3, names the file as the reference does */
+  int32_t elem, nid;
+  double *weights;            /* [npe]: basis values at the point, or a unit weight at the nearest node */
+  double mapped[2];           /* nearest-node or interpolated coordinate */
+  char filename[1024];
+  int header_written;
+};
+
 #endif

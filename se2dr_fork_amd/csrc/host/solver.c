@@ -22,6 +22,10 @@ struct se2w_solver {
   int last_n;
   double *times, *trace, *psi;
   size_t times_cap, trace_cap, psi_cap;
+  int nuv;            /* single-station receivers */
+  double *uv_weights; /* [nuv][npe] */
+  double *uv_trace;   /* [steps][nuv][4] */
+  size_t uv_cap;
 };
 
 static int gpu_fail(int code)
@@ -104,6 +108,7 @@ int se2w_solver_destroy(SE2WSolver *sp)
   if ((*sp)->ctx) se2w_destroy((*sp)->ctx);
   free((*sp)->src); free((*sp)->stf);
   free((*sp)->times); free((*sp)->trace); free((*sp)->psi);
+  free((*sp)->uv_weights); free((*sp)->uv_trace);
   free(*sp);
   *sp = NULL;
   return 0;
@@ -120,6 +125,44 @@ static int grow(double **p, size_t *cap, size_t n)
   *p = (double *)malloc(sizeof(double) * n);
   if (!*p) { *cap = 0; return host_fail(SE2W_HOST_ERR_MEM, "out of memory"); }
   *cap = n;
+  return 0;
+}
+
+/* Single-station rows of the last run (weighted sums over element nodes). */
+static int collect_uv(struct se2w_solver *sv, int nsteps)
+{
+  int ierr;
+  if (sv->nuv == 0) return 0;
+  if ((ierr = grow(&sv->uv_trace, &sv->uv_cap, (size_t)nsteps * sv->nuv * 4))) return ierr;
+  if ((ierr = se2w_get_element_receivers(sv->ctx, sv->uv_weights, sv->uv_trace))) return gpu_fail(ierr);
+  return 0;
+}
+
+int se2w_solver_set_uv_receivers(SE2WSolver sv, int n, SE2WReceiverUV *rs)
+{
+  int32_t *elem;
+  double *w;
+  int i, ierr;
+  if (!sv || n < 0 || (n > 0 && !rs)) return host_fail(SE2W_HOST_ERR_ARG, "bad receiver arguments");
+  elem = (int32_t *)malloc(sizeof(int32_t) * (n > 0 ? n : 1));
+  w = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1) * sv->spec->npe);
+  if (!elem || !w) { free(elem); free(w); return host_fail(SE2W_HOST_ERR_MEM, "out of memory"); }
+  for (i = 0; i < n; i++) {
+    if (!rs[i] || rs[i]->spec != sv->spec) { free(elem); free(w); return host_fail(SE2W_HOST_ERR_ARG, "receiver %d does not belong to the solver's mesh", i); }
+    se2w_receiver_uv_get(rs[i], &elem[i], &w[(size_t)i * sv->spec->npe], NULL);
+  }
+  if ((ierr = se2w_set_element_receivers(sv->ctx, n, elem))) { free(elem); free(w); return gpu_fail(ierr); }
+  free(elem);
+  free(sv->uv_weights);
+  sv->uv_weights = w;
+  sv->nuv = n;
+  return 0;
+}
+
+int se2w_solver_get_uv_trace(SE2WSolver sv, double *trace)
+{
+  if (!sv) return host_fail(SE2W_HOST_ERR_ARG, "null solver");
+  if (trace && sv->nuv > 0 && sv->last_n) memcpy(trace, sv->uv_trace, sizeof(double) * (size_t)sv->last_n * sv->nuv * 4);
   return 0;
 }
 
@@ -141,6 +184,7 @@ int se2w_solver_advance(SE2WSolver sv, int nsteps, double dt)
   }
   if ((ierr = se2w_run(sv->ctx, nsteps, dt, sv->psi))) return gpu_fail(ierr);
   if (nr > 0 && (ierr = se2w_get_receivers(sv->ctx, sv->trace))) return gpu_fail(ierr);
+  if ((ierr = collect_uv(sv, nsteps))) return ierr;
   sv->time = t;
   sv->steps += nsteps;
   sv->last_n = nsteps;
@@ -193,6 +237,7 @@ int se2w_solvers_advance(int n, SE2WSolver *sv, se2w_ensemble *ens, int nsteps, 
     struct se2w_solver *v = sv[i];
     const int nr = v->recv ? v->recv->nr : 0;
     if (nr > 0 && (ierr = se2w_get_receivers(v->ctx, v->trace))) { ierr = gpu_fail(ierr); goto done; }
+    if ((ierr = collect_uv(v, nsteps))) goto done;
     v->time = v->times[nsteps - 1];
     v->steps += nsteps;
     v->last_n = nsteps;

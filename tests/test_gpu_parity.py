@@ -285,3 +285,35 @@ def test_mfma_kernel_is_rejected_where_not_built():
     b64 = SC.build(SC.small(mx=5, my=4, p=6), precision=S.FP64)
     with pytest.raises(S.Se2wError, match="matrix-core"):
         b64.solver.ctx.set_kernel(S.KERNEL_MFMA)
+
+
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32])
+def test_single_station_receivers_sample_the_state(prec):
+    """RecordUV / RecordUV_interp on the GPU: each step's row is the weighted
+    sum, in the reference's node order, of the u and v that step leaves."""
+    sc = SC.small(mx=9, my=7, p=5, nt=10, seed=44)
+    b = SC.build(sc, precision=prec)
+    stations = [S.ReceiverUV(b.spec, [1500.3, 2200.7], interpolate=False),
+                S.ReceiverUV(b.spec, [1500.3, 2200.7], interpolate=True),
+                S.ReceiverUV(b.spec, [3110.0, 640.2], interpolate=True)]
+    b.solver.set_uv_receivers(stations)
+    el = b.spec.element_map()
+    for step in range(4):
+        b.solver.advance(1, b.dt)
+        rows = b.solver.uv_trace(1)[0]
+        u, v, _ = b.solver.ctx.get_state()
+        for r, st in enumerate(stations):
+            e, w, _ = st.get()
+            want = [0.0, 0.0, 0.0, 0.0]
+            for q, n in enumerate(el[e]):
+                vals = (u[2 * n], u[2 * n + 1], v[2 * n], v[2 * n + 1])
+                for f in range(4):
+                    want[f] = want[f] + float(w[q]) * float(vals[f])
+            if prec == S.FP64:
+                assert list(rows[r]) == want
+            else:
+                np.testing.assert_allclose(rows[r], want, rtol=1e-6, atol=1e-6 * np.abs(want).max() + 1e-300)
+    # the nearest-node station returns that node's values exactly
+    e, w, _ = stations[0].get()
+    n = el[e][int(np.argmax(w))]
+    assert list(b.solver.uv_trace(1)[0][0]) == [u[2 * n], u[2 * n + 1], v[2 * n], v[2 * n + 1]]

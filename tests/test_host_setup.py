@@ -180,3 +180,42 @@ def test_perturbed_material_follows_the_reference_draws():
     np.testing.assert_array_equal(r, rho)
     np.testing.assert_array_equal(mu, vs * vs * rho)
     np.testing.assert_array_equal(lam, vp * vp * rho - 2.0 * (vs * vs * rho))
+
+
+def test_single_station_receivers(tmp_path):
+    """RecordUV / RecordUV_interp set-up (se2wave.c:2351-2581): the nearest
+    node by brute force, interpolation weights that reproduce the station's
+    coordinates and sum to one, and the reference's file names and headers."""
+    sc = SC.small(mx=6, my=5, p=4)
+    b = SC.build(sc, with_solver=False)
+    coords = b.spec.coords()
+    x, y = coords[0::2], coords[1::2]
+    el = b.spec.element_map()
+    xr = np.array([1234.5, 2987.6])
+    near = S.ReceiverUV(b.spec, xr, interpolate=False, source_implementation=1)
+    e, w, m = near.get()
+    nodes = el[e]
+    d2 = (x[nodes] - xr[0]) ** 2 + (y[nodes] - xr[1]) ** 2
+    k = int(np.argmin(d2))
+    assert w[k] == 1.0 and w.sum() == 1.0 and (m == [x[nodes[k]], y[nodes[k]]]).all()
+    interp = S.ReceiverUV(b.spec, xr, interpolate=True, source_implementation=-1)
+    e2, w2, m2 = interp.get()
+    assert e2 == e
+    assert abs(w2.sum() - 1.0) < 1e-13
+    np.testing.assert_allclose(m2, xr, rtol=1e-12)
+    np.testing.assert_allclose([w2 @ x[nodes], w2 @ y[nodes]], xr, rtol=1e-12)
+    for r in (near, interp):
+        r.write_header(str(tmp_path))
+        r.append_rows([0.001, 0.002], [[1.0, -2.0, 3.0, -4.0], [0.5, 0.25, -0.125, 1e-20]])
+    a = (tmp_path / "closestqpsource-receiverCP-6x5-p4.dat").read_text().splitlines()
+    assert a[0] == "# SpecFECtx meta data" and a[1] == "#   mx 6 : my 5 : basis order 4"
+    assert a[2] == "#   source implementation 1"
+    assert a[5] == "#   + takes displ/velo from basis nearest to requested receiver location"
+    assert a[6].startswith("#   + receiver location: x,y +1.23450000e+03 +2.98760000e+03 --mapped to nearest node --> ")
+    assert a[8] == "#   time ux uy vx vy"
+    assert a[9] == "1.0000e-03 +1.00000000e+00 -2.00000000e+00 +3.00000000e+00 -4.00000000e+00"
+    bb = (tmp_path / "defaultsource-receiver-6x5-p4.dat").read_text().splitlines()
+    assert bb[5] == "#   + records displ/velo at requested receiver location through interpolating the FE solution"
+    assert bb[6] == "# Time series header" and len(bb) == 10
+    with pytest.raises(S.Se2wError, match="no containing element"):
+        S.ReceiverUV(b.spec, [4000.0, 100.0])  # on the right boundary: truncation gives element mx
