@@ -403,9 +403,12 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   constexpr int TEX = TileShape<P, R>::TEX, TEY = TileShape<P, R>::TEY;
   constexpr int NXL = (TEX + 1) * P + 1, NYL = (TEY + 1) * P + 1;
   constexpr int QX = (TEX + 1) * NB, QY = (TEY + 1) * NB;
+  constexpr int NE = (TEX + 1) * (TEY + 1);
   __shared__ R su[2][NYL][NXL];
   __shared__ R ss[3][QY][QX];
   __shared__ R sD[NB * NB], sW[NB * NB];
+  // per tile element (+ halo): 1/J00, 1/J11, J00 J11, lambda, mu
+  __shared__ R econ[5][NE];
 
   const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
   const int ex0 = tx * TEX, ey0 = ty * TEY;
@@ -415,6 +418,18 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   for (int k = tid; k < NB * NB; k += kThreads) {
     sD[k] = prm.D[k];
     sW[k] = prm.w2[k];
+  }
+  for (int k = tid; k < NE; k += kThreads) {
+    const int ex = ex0 - 1 + k % (TEX + 1), ey = ey0 - 1 + k / (TEX + 1);
+    if (ex >= 0 && ex < prm.mx && ey >= 0 && ey < prm.my) {
+#pragma clang fp contract(off)
+      const R J00 = R(0.5) * prm.elem_dx[ex], J11 = R(0.5) * prm.elem_dy[ey];
+      econ[0][k] = R(1) / J00;
+      econ[1][k] = R(1) / J11;
+      econ[2][k] = J00 * J11;
+      econ[3][k] = prm.lambda[ex + ey * prm.mx];
+      econ[4][k] = prm.mu[ex + ey * prm.mx];
+    }
   }
   // Phase A: displacement of the tile's nodes (+ halo) into LDS.
   for (int k = tid; k < NXL * NYL; k += kThreads) {
@@ -427,28 +442,25 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   }
   __syncthreads();
   // Phase B: stress at the GLL points of the tile elements (+ halo).
-  for (int k = tid; k < QX * QY; k += kThreads) {
-    const int qy = k / QX, qx = k - qy * QX;
-    const int elx = qx / NB, qi = qx - elx * NB;
-    const int ely = qy / NB, qj = qy - ely * NB;
-    const int ex = ex0 - 1 + elx, ey = ey0 - 1 + ely;
+  for (unsigned k = tid; k < (unsigned)(QX * QY); k += kThreads) {
+    const unsigned qy = k / QX, qx = k - qy * QX;
+    const unsigned elx = qx / NB, qi = qx - elx * NB;
+    const unsigned ely = qy / NB, qj = qy - ely * NB;
+    const int ex = ex0 - 1 + (int)elx, ey = ey0 - 1 + (int)ely;
     if (ex < 0 || ex >= prm.mx || ey < 0 || ey >= prm.my) continue;
-    const int e = ex + ey * prm.mx;
-    const int lx0 = elx * P, ly0 = ely * P;
+    const unsigned ei = ely * (TEX + 1) + elx;
+    const unsigned lx0 = elx * P, ly0 = ely * P;
     const R *uxr = &su[0][ly0 + qj][lx0];
     const R *uyr = &su[1][ly0 + qj][lx0];
     const R *uxc = &su[0][ly0][lx0 + qi];
     const R *uyc = &su[1][ly0][lx0 + qi];
     R sxx, syy, sxy;
     if constexpr (MODE == MODE_F64) {
-#pragma clang fp contract(off)
-      const double J00 = 0.5 * prm.elem_dx[ex], J11 = 0.5 * prm.elem_dy[ey];
-      const double iJ00 = 1.0 / J00, iJ11 = 1.0 / J11;
-      stress_f64<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, iJ00, iJ11, prm.lambda[e], prm.mu[e], sxx, syy, sxy);
+      stress_f64<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, econ[0][ei], econ[1][ei], econ[3][ei], econ[4][ei], sxx, syy,
+                    sxy);
     } else {
-      const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
-      const float fac = (J00 * J11) * sW[qi + qj * NB];
-      stress_f32<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, 1.f / J00, 1.f / J11, prm.lambda[e], prm.mu[e], fac, sxx,
+      const float fac = econ[2][ei] * sW[qi + qj * NB];
+      stress_f32<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, econ[0][ei], econ[1][ei], econ[3][ei], econ[4][ei], fac, sxx,
                     syy, sxy);
     }
     ss[0][qy][qx] = sxx;
@@ -486,16 +498,14 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
         const R *Syy = &ss[1][ely * NB][elx * NB];
         const R *Sxy = &ss[2][ely * NB][elx * NB];
         R cfx, cfy;
+        const int ei = ely * (TEX + 1) + elx;
         if constexpr (MODE == MODE_F64) {
 #pragma clang fp contract(off)
-          const double J00 = 0.5 * prm.elem_dx[ex], J11 = 0.5 * prm.elem_dy[ey];
-          const double iJ00 = 1.0 / J00, iJ11 = 1.0 / J11, detJ = J00 * J11;
-          force_f64<P>(Sxx, Syy, Sxy, QX, i, j, sD, sW, iJ00, iJ11, detJ, cfx, cfy);
+          force_f64<P>(Sxx, Syy, Sxy, QX, i, j, sD, sW, econ[0][ei], econ[1][ei], econ[2][ei], cfx, cfy);
           fx = fx + cfx;
           fy = fy + cfy;
         } else {
-          const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
-          force_f32<P>(Sxx, Syy, Sxy, QX, i, j, sD, 1.f / J00, 1.f / J11, cfx, cfy);
+          force_f32<P>(Sxx, Syy, Sxy, QX, i, j, sD, econ[0][ei], econ[1][ei], cfx, cfy);
           fx += cfx;
           fy += cfy;
         }
