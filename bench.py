@@ -92,6 +92,8 @@ def kernel_signature(variant, p, mode):
         return f"tile_kernel<{p}, {mode}, true>"
     if variant == 9:
         return f"mfma_kernel<{p}, {mode}>"
+    if variant == 10:
+        return f"line_kernel<{p}, {mode}>"
     ex, ey, w = REG_SCHEDULES[variant - 1]
     return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}>"
 
@@ -102,12 +104,15 @@ def ens_kernel_signature(variant, p, mode):
         return f"ens_tile_kernel<{p}, {mode}>"
     if variant == 9:
         return f"ens_mfma_kernel<{p}, {mode}>"
+    if variant == 10:
+        return f"ens_line_kernel<{p}, {mode}>"
     ex, ey, w = REG_SCHEDULES[variant - 1]
     return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}>"
 
 
 def kernel_name(variant, p, mode):
     what = ("LDS-staged tile" if variant == 0 else "matrix-core contractions" if variant == 9 else
+            "line contractions" if variant == 10 else
             "thread per element, %dx%d element threads" % REG_SCHEDULES[variant - 1][:2])
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
 

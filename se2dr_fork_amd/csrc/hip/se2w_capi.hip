@@ -266,7 +266,7 @@ void build_tile_lists(const se2w_ctx *c, const std::vector<int> &elem, std::vect
 
 void set_tiling(se2w_ctx *c) {
   if (is_reg_variant(c->variant)) reg_shape(c->variant, c->tex, c->tey);
-  else if (c->variant == SE2W_KERNEL_MFMA) mfma_shape(c->p, c->tex, c->tey);
+  else if (c->variant == SE2W_KERNEL_MFMA || c->variant == SE2W_KERNEL_LINE) mfma_shape(c->p, c->tex, c->tey);
   else tile_shape(c->p, c->mode, c->tex, c->tey);
   c->tiles_x = (c->mx + c->tex - 1) / c->tex;
   c->tiles_y = (c->my + c->tey - 1) / c->tey;
@@ -344,6 +344,10 @@ void launch_element_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &p
   if constexpr (MODE != MODE_F64 && P >= 6 && FUSED) {
     if (c->variant == SE2W_KERNEL_MFMA) {
       hipLaunchKernelGGL((mfma_kernel<P, MODE>), grid, dim3(kThreads), 0, c->stream, prm);
+      return;
+    }
+    if (c->variant == SE2W_KERNEL_LINE) {
+      hipLaunchKernelGGL((line_kernel<P, MODE>), grid, dim3(kThreads), 0, c->stream, prm);
       return;
     }
   }
@@ -977,10 +981,12 @@ int se2w_get_timing(se2w_ctx *c, double *run_ms, double kernel_ms[2]) {
 
 int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
-  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant) && variant != SE2W_KERNEL_MFMA)
+  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant) && variant != SE2W_KERNEL_MFMA && variant != SE2W_KERNEL_LINE)
     return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
   if (variant == SE2W_KERNEL_MFMA && !mfma_available(c->p, c->mode, c->schedule))
     return fail(SE2W_ERR_UNSUPPORTED, "matrix-core kernel needs FP32, 6 <= p <= 8 and the fused schedule");
+  if (variant == SE2W_KERNEL_LINE && !mfma_available(c->p, c->mode, c->schedule))
+    return fail(SE2W_ERR_UNSUPPORTED, "line kernel needs FP32, 6 <= p <= 8 and the fused schedule");
   if (is_reg_variant(variant) && !reg_available(c->p, c->mode))
     return fail(SE2W_ERR_UNSUPPORTED, "register-blocked kernel needs FP32 and 2 <= p <= 6");
   if (is_reg_variant(variant) && !reg_variant_built(c->p, variant, c->schedule))
@@ -1006,7 +1012,7 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   return SE2W_OK;
 }
 
-int se2w_kernel_count(void) { return SE2W_KERNEL_MFMA + 1; }
+int se2w_kernel_count(void) { return SE2W_KERNEL_LINE + 1; }
 
 // Autotuner: times every variant built for the context's (p, precision,
 // schedule) on the context's own mesh and keeps the fastest.  The state is
@@ -1087,7 +1093,10 @@ int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_varian
   if (reg_available(c->p, c->mode))
     for (int k = 0; k < kNumRegSchedules; k++)
       if (reg_variant_built(c->p, SE2W_KERNEL_REG + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG + k);
-  if (mfma_available(c->p, c->mode, c->schedule)) cands.push_back(SE2W_KERNEL_MFMA);
+  if (mfma_available(c->p, c->mode, c->schedule)) {
+    cands.push_back(SE2W_KERNEL_MFMA);
+    cands.push_back(SE2W_KERNEL_LINE);
+  }
   Snapshot sn;
   if (int ierr = snapshot_take(c, sn)) return ierr;
   const int variant0 = c->variant;
@@ -1368,7 +1377,7 @@ int se2w_ensemble_autotune(se2w_ensemble *e, int32_t steps, int32_t reps, int32_
   if (!e || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
   HIPCHK(hipSetDevice(e->device));
   std::vector<int> cands;
-  for (int v = 0; v <= SE2W_KERNEL_MFMA; v++)
+  for (int v = 0; v <= SE2W_KERNEL_LINE; v++)
     if (ens_variant_built(e, v)) cands.push_back(v);
   const size_t n = e->members.size();
   std::vector<Snapshot> sn(n);

@@ -49,7 +49,12 @@ void ens_reg_kernel(EnsArgs<float> a) {
 
 template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void ens_mfma_kernel(EnsArgs<float> a) {
-  ens_loop(a, [](const StepParams<float> &prm, int tile) { mfma_body<P, MODE>(prm, tile); });
+  ens_loop(a, [](const StepParams<float> &prm, int tile) { mfma_body<P, MODE, false>(prm, tile); });
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void ens_line_kernel(EnsArgs<float> a) {
+  ens_loop(a, [](const StepParams<float> &prm, int tile) { mfma_body<P, MODE, true>(prm, tile); });
 }
 
 namespace {
@@ -93,8 +98,8 @@ int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc
     return 0;
   }
   if constexpr (MODE != MODE_F64 && P >= 6) {
-    if (variant == SE2W_KERNEL_MFMA) {
-      auto k = ens_mfma_kernel<P, MODE>;
+    if (variant == SE2W_KERNEL_MFMA || variant == SE2W_KERNEL_LINE) {
+      auto k = variant == SE2W_KERNEL_MFMA ? ens_mfma_kernel<P, MODE> : ens_line_kernel<P, MODE>;
       if (!args) {
         int n = 0;
         if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kThreads, 0) != hipSuccess) return -1;

@@ -85,8 +85,29 @@ __device__ __forceinline__ void mfma_phase(int wave, const float (&A)[3], Fetch 
   }
 }
 
-template <int P, int MODE>
+// The same products on the vector ALUs, one line (one N column) per thread:
+// out[m] = sum_k A[m][k] in[k] for m, k < NB, with A = D or D^T read from the
+// uniform table (scalar loads) and the NB inputs and outputs in registers.
+template <int NB, int NLINES, bool TRANS, typename Load, typename Store>
+__device__ __forceinline__ void line_phase(const float *__restrict__ D, Load load, Store store) {
+  for (int line = threadIdx.x; line < NLINES; line += kThreads) {
+    float in[NB];
+#pragma unroll
+    for (int k = 0; k < NB; k++) in[k] = load(line, k);
+#pragma unroll
+    for (int m = 0; m < NB; m++) {
+      float acc = 0.f;
+#pragma unroll
+      for (int k = 0; k < NB; k++) acc = fmaf(TRANS ? D[k * NB + m] : D[m * NB + k], in[k], acc);
+      store(line, m, acc);
+    }
+  }
+}
+
+template <int P, int MODE, bool LINES>
 __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile) {
+  // LINES = false: contractions on the matrix cores; true: on the vector ALUs,
+  // one line per thread (same data layout and phases).
   static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
   using Lay = MfmaLayout<P>;
   constexpr int NB = Lay::NB, TEX = Lay::TEX, TEY = Lay::TEY;
@@ -149,6 +170,18 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
   for (int k = tid; k < NB * NB; k += kThreads) sW[k] = prm.w2[k];
   __syncthreads();
   // B1: xi-derivatives along every node row, per element column.
+  if constexpr (LINES) {
+    line_phase<NB, 2 * (TEX + 1) * NYL, false>(
+        prm.D,
+        [&](int line, int k) {
+          const int row = line % NYL, ce = line / NYL, el = ce % (TEX + 1), c = ce / (TEX + 1);
+          return su[c][row][el * P + k];
+        },
+        [&](int line, int m, float v) {
+          const int row = line % NYL, ce = line / NYL, el = ce % (TEX + 1), c = ce / (TEX + 1);
+          Dx[(c * NYL + row) * QX + el * NB + m] = v;
+        });
+  } else {
   mfma_phase<Lay::JB1, 4>(
       wave, Ad,
       [&](int job, int s) {
@@ -165,7 +198,20 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
           if (qi < NB && row < NYL) Dx[(c * NYL + row) * QX + el * NB + qi] = acc[r];
         }
       });
+  }
   // B2: eta-derivatives along every node column, per element row.
+  if constexpr (LINES) {
+    line_phase<NB, 2 * (TEY + 1) * NXL, false>(
+        prm.D,
+        [&](int line, int k) {
+          const int col = line % NXL, ce = line / NXL, er = ce % (TEY + 1), c = ce / (TEY + 1);
+          return su[c][er * P + k][col];
+        },
+        [&](int line, int m, float v) {
+          const int col = line % NXL, ce = line / NXL, er = ce % (TEY + 1), c = ce / (TEY + 1);
+          Dy[(c * QY + er * NB + m) * NXL + col] = v;
+        });
+  } else {
   mfma_phase<Lay::JB2, 4>(
       wave, Ad,
       [&](int job, int s) {
@@ -182,6 +228,7 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
           if (qj < NB && col < NXL) Dy[(c * QY + er * NB + qj) * NXL + col] = acc[r];
         }
       });
+  }
   __syncthreads();
   // B3: weighted stresses at the quadrature points.
   for (int pnt = tid; pnt < Lay::PTS; pnt += kThreads) {
@@ -199,6 +246,18 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
   }
   __syncthreads();  // S complete; the derivatives are dead, R and C reuse their space
   // C1: row terms R(c, el, i, qrow); c = 0 from sxx (x force), c = 1 from sxy (y force)
+  if constexpr (LINES) {
+    line_phase<NB, 2 * (TEX + 1) * QY, true>(
+        prm.D,
+        [&](int line, int k) {
+          const int qrow = line % QY, ce = line / QY, el = ce % (TEX + 1), c = ce / (TEX + 1);
+          return S[(c == 0 ? 0 : 2) * QY * QX + qrow * QX + el * NB + k];
+        },
+        [&](int line, int m, float v) {
+          const int qrow = line % QY, ce = line / QY, el = ce % (TEX + 1), c = ce / (TEX + 1);
+          Rr[(c * QX + el * NB + m) * QY + qrow] = v;
+        });
+  } else {
   mfma_phase<Lay::JC1, 4>(
       wave, At,
       [&](int job, int s) {
@@ -216,7 +275,20 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
           if (i < NB && qrow < QY) Rr[(c * QX + el * NB + i) * QY + qrow] = acc[r];
         }
       });
+  }
   // C2: column terms C(c, er, j, qcol); c = 0 from sxy (x force), c = 1 from syy (y force)
+  if constexpr (LINES) {
+    line_phase<NB, 2 * (TEY + 1) * QX, true>(
+        prm.D,
+        [&](int line, int k) {
+          const int qcol = line % QX, ce = line / QX, er = ce % (TEY + 1), c = ce / (TEY + 1);
+          return S[(c == 0 ? 2 : 1) * QY * QX + (er * NB + k) * QX + qcol];
+        },
+        [&](int line, int m, float v) {
+          const int qcol = line % QX, ce = line / QX, er = ce % (TEY + 1), c = ce / (TEY + 1);
+          Cc[(c * QY + er * NB + m) * QX + qcol] = v;
+        });
+  } else {
   mfma_phase<Lay::JC2, 4>(
       wave, At,
       [&](int job, int s) {
@@ -234,6 +306,7 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
           if (j < NB && qcol < QX) Cc[(c * QY + er * NB + j) * QX + qcol] = acc[r];
         }
       });
+  }
   __syncthreads();
   // D: owned nodes: element contributions in ascending element order, update.
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
@@ -277,7 +350,14 @@ template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void mfma_kernel(StepParams<float> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  mfma_body<P, MODE>(prm, tile);
+  mfma_body<P, MODE, false>(prm, tile);
+}
+
+template <int P, int MODE>
+__global__ __launch_bounds__(kThreads) void line_kernel(StepParams<float> prm) {
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
+  mfma_body<P, MODE, true>(prm, tile);
 }
 
 }  // namespace se2w

@@ -272,10 +272,11 @@ def test_mfma_kernel_matches_the_vector_kernels(p):
     _, _, r64, u64, _, _ = gpu_run(sc, precision=S.FP64)
     for prec in [S.FP32, S.FP32_KAHAN]:
         _, _, r0, u0, v0, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_LDS)
-        _, _, r1, u1, v1, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_MFMA)
-        assert rel_l2(u1, u0) < 2e-6 and rel_l2(v1, v0) < 2e-6
-        assert rel_l2(r1[..., :4], r64[..., :4]) < 5e-6
-        assert rel_l2(u1, u64) < 3 * max(rel_l2(u0, u64), 1e-7)
+        for kern in (S.KERNEL_MFMA, S.KERNEL_LINE):
+            _, _, r1, u1, v1, _ = gpu_run(sc, precision=prec, kernel=kern)
+            assert rel_l2(u1, u0) < 2e-6 and rel_l2(v1, v0) < 2e-6
+            assert rel_l2(r1[..., :4], r64[..., :4]) < 5e-6
+            assert rel_l2(u1, u64) < 3 * max(rel_l2(u0, u64), 1e-7)
 
 
 def test_mfma_kernel_is_rejected_where_not_built():
