@@ -94,6 +94,8 @@ def kernel_signature(variant, p, mode):
         return f"mfma_kernel<{p}, {mode}>"
     if variant == 10:
         return f"line_kernel<{p}, {mode}>"
+    if variant == 11:
+        return f"tile_kernel<{p}, {mode}, true, 512>"
     ex, ey, w = REG_SCHEDULES[variant - 1]
     return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}>"
 
@@ -101,18 +103,20 @@ def kernel_signature(variant, p, mode):
 def ens_kernel_signature(variant, p, mode):
     """The batched (ensemble) step kernel's template signature."""
     if variant == 0:
-        return f"ens_tile_kernel<{p}, {mode}>"
+        return f"ens_tile_kernel<{p}, {mode}, 256>"
     if variant == 9:
         return f"ens_mfma_kernel<{p}, {mode}>"
     if variant == 10:
         return f"ens_line_kernel<{p}, {mode}>"
+    if variant == 11:
+        return f"ens_tile_kernel<{p}, {mode}, 512>"
     ex, ey, w = REG_SCHEDULES[variant - 1]
     return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}>"
 
 
 def kernel_name(variant, p, mode):
     what = ("LDS-staged tile" if variant == 0 else "matrix-core contractions" if variant == 9 else
-            "line contractions" if variant == 10 else
+            "line contractions" if variant == 10 else "LDS-staged tile, 512 threads" if variant == 11 else
             "thread per element, %dx%d element threads" % REG_SCHEDULES[variant - 1][:2])
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
 

@@ -351,6 +351,12 @@ void launch_element_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &p
       return;
     }
   }
+  if constexpr (FUSED) {
+    if (c->variant == SE2W_KERNEL_LDS_WIDE) {
+      hipLaunchKernelGGL((tile_kernel<P, MODE, true, 512>), grid, dim3(512), 0, c->stream, prm);
+      return;
+    }
+  }
   hipLaunchKernelGGL((tile_kernel<P, MODE, FUSED>), grid, dim3(kThreads), 0, c->stream, prm);
 }
 
@@ -981,8 +987,11 @@ int se2w_get_timing(se2w_ctx *c, double *run_ms, double kernel_ms[2]) {
 
 int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
-  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant) && variant != SE2W_KERNEL_MFMA && variant != SE2W_KERNEL_LINE)
+  if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant) && variant != SE2W_KERNEL_MFMA &&
+      variant != SE2W_KERNEL_LINE && variant != SE2W_KERNEL_LDS_WIDE)
     return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
+  if (variant == SE2W_KERNEL_LDS_WIDE && c->schedule != SE2W_SCHED_FUSED)
+    return fail(SE2W_ERR_UNSUPPORTED, "the 512-thread tile kernel is built for the fused schedule");
   if (variant == SE2W_KERNEL_MFMA && !mfma_available(c->p, c->mode, c->schedule))
     return fail(SE2W_ERR_UNSUPPORTED, "matrix-core kernel needs FP32, 6 <= p <= 8 and the fused schedule");
   if (variant == SE2W_KERNEL_LINE && !mfma_available(c->p, c->mode, c->schedule))
@@ -1012,7 +1021,7 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   return SE2W_OK;
 }
 
-int se2w_kernel_count(void) { return SE2W_KERNEL_LINE + 1; }
+int se2w_kernel_count(void) { return SE2W_KERNEL_LDS_WIDE + 1; }
 
 // Autotuner: times every variant built for the context's (p, precision,
 // schedule) on the context's own mesh and keeps the fastest.  The state is
@@ -1090,6 +1099,7 @@ int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_varian
     }
   }
   std::vector<int> cands = {SE2W_KERNEL_LDS};
+  if (c->schedule == SE2W_SCHED_FUSED) cands.push_back(SE2W_KERNEL_LDS_WIDE);
   if (reg_available(c->p, c->mode))
     for (int k = 0; k < kNumRegSchedules; k++)
       if (reg_variant_built(c->p, SE2W_KERNEL_REG + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG + k);
@@ -1377,7 +1387,7 @@ int se2w_ensemble_autotune(se2w_ensemble *e, int32_t steps, int32_t reps, int32_
   if (!e || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
   HIPCHK(hipSetDevice(e->device));
   std::vector<int> cands;
-  for (int v = 0; v <= SE2W_KERNEL_LINE; v++)
+  for (int v = 0; v <= SE2W_KERNEL_LDS_WIDE; v++)
     if (ens_variant_built(e, v)) cands.push_back(v);
   const size_t n = e->members.size();
   std::vector<Snapshot> sn(n);

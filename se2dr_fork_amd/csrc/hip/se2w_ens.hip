@@ -36,9 +36,9 @@ __device__ __forceinline__ void ens_loop(const EnsArgs<R> &a, Body body) {
   }
 }
 
-template <int P, int MODE>
-__global__ __launch_bounds__(kThreads) void ens_tile_kernel(EnsArgs<typename RealOf<MODE>::T> a) {
-  ens_loop(a, [](const StepParams<typename RealOf<MODE>::T> &prm, int tile) { tile_body<P, MODE, true>(prm, tile); });
+template <int P, int MODE, int NT>
+__global__ __launch_bounds__(NT) void ens_tile_kernel(EnsArgs<typename RealOf<MODE>::T> a) {
+  ens_loop(a, [](const StepParams<typename RealOf<MODE>::T> &prm, int tile) { tile_body<P, MODE, true, NT>(prm, tile); });
 }
 
 template <int P, int MODE, int EX, int EY, int WAVES>
@@ -85,16 +85,17 @@ int reg_dispatch(int sched, const void *args, int grid, hipStream_t s, int *bpc,
 template <int P, int MODE>
 int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
   using R = typename RealOf<MODE>::T;
-  if (variant == SE2W_KERNEL_LDS) {
-    auto k = ens_tile_kernel<P, MODE>;
+  if (variant == SE2W_KERNEL_LDS || variant == SE2W_KERNEL_LDS_WIDE) {
+    const int nt = variant == SE2W_KERNEL_LDS ? kThreads : 512;
+    auto k = variant == SE2W_KERNEL_LDS ? ens_tile_kernel<P, MODE, kThreads> : ens_tile_kernel<P, MODE, 512>;
     if (!args) {
       int n = 0;
-      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, kThreads, 0) != hipSuccess) return -1;
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, 0) != hipSuccess) return -1;
       *bpc = n;
-      *thr = kThreads;
+      *thr = nt;
       return 0;
     }
-    hipLaunchKernelGGL(k, dim3(grid), dim3(kThreads), 0, s, *(const EnsArgs<R> *)args);
+    hipLaunchKernelGGL(k, dim3(grid), dim3(nt), 0, s, *(const EnsArgs<R> *)args);
     return 0;
   }
   if constexpr (MODE != MODE_F64 && P >= 6) {

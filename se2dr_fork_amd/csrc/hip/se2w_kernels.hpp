@@ -395,7 +395,7 @@ __device__ __forceinline__ void node_update(const StepParams<typename RealOf<MOD
 // A workgroup that calls it for several tiles in a row must put a barrier
 // between the calls: the last phase of one call reads `su`, which the first
 // phase of the next call overwrites.
-template <int P, int MODE, bool FUSED>
+template <int P, int MODE, bool FUSED, int NT = kThreads>
 __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>::T> &prm, int tile) {
   using R = typename RealOf<MODE>::T;
   using R2 = typename V2<R>::T;
@@ -415,11 +415,11 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   const int gx0 = (ex0 - 1) * P, gy0 = (ey0 - 1) * P;
   const int tid = threadIdx.x;
 
-  for (int k = tid; k < NB * NB; k += kThreads) {
+  for (int k = tid; k < NB * NB; k += NT) {
     sD[k] = prm.D[k];
     sW[k] = prm.w2[k];
   }
-  for (int k = tid; k < NE; k += kThreads) {
+  for (int k = tid; k < NE; k += NT) {
     const int ex = ex0 - 1 + k % (TEX + 1), ey = ey0 - 1 + k / (TEX + 1);
     if (ex >= 0 && ex < prm.mx && ey >= 0 && ey < prm.my) {
 #pragma clang fp contract(off)
@@ -432,7 +432,7 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
     }
   }
   // Phase A: displacement of the tile's nodes (+ halo) into LDS.
-  for (int k = tid; k < NXL * NYL; k += kThreads) {
+  for (int k = tid; k < NXL * NYL; k += NT) {
     const int ly = k / NXL, lx = k - ly * NXL;
     const int gx = gx0 + lx, gy = gy0 + ly;
     R2 uv = R2{R(0), R(0)};
@@ -442,7 +442,7 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   }
   __syncthreads();
   // Phase B: stress at the GLL points of the tile elements (+ halo).
-  for (unsigned k = tid; k < (unsigned)(QX * QY); k += kThreads) {
+  for (unsigned k = tid; k < (unsigned)(QX * QY); k += NT) {
     const unsigned qy = k / QX, qx = k - qy * QX;
     const unsigned elx = qx / NB, qi = qx - elx * NB;
     const unsigned ely = qy / NB, qj = qy - ely * NB;
@@ -472,7 +472,7 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
-  for (int k = tid; k < OX * OY; k += kThreads) {
+  for (int k = tid; k < OX * OY; k += NT) {
     const int oy = k / OX, ox = k - oy * OX;
     const int gx = ex0 * P + ox, gy = ey0 * P + oy;
     int exs[2], eys[2], nex = 0, ney = 0;
@@ -520,11 +520,12 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   }
 }
 
-template <int P, int MODE, bool FUSED>
-__global__ __launch_bounds__(kThreads) void tile_kernel(StepParams<typename RealOf<MODE>::T> prm) {
+// NT threads per workgroup: 256, or 512 for more waves in flight per tile.
+template <int P, int MODE, bool FUSED, int NT = kThreads>
+__global__ __launch_bounds__(NT) void tile_kernel(StepParams<typename RealOf<MODE>::T> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  tile_body<P, MODE, FUSED>(prm, tile);
+  tile_body<P, MODE, FUSED, NT>(prm, tile);
 }
 
 // Update kernel of the split schedule: one workgroup per tile of owned nodes.

@@ -164,7 +164,7 @@ def test_fp32_kernel_variants_agree():
     for prec in [S.FP32, S.FP32_KAHAN]:
         _, _, r0, u0, _, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_LDS)
         ref = None
-        for k in range(S.KERNEL_REG, hip.se2w_kernel_count()):
+        for k in range(S.KERNEL_REG, S.KERNEL_MFMA):
             try:
                 _, _, r, u, v, a = gpu_run(sc, precision=prec, kernel=k)
             except S.Se2wError:
@@ -179,14 +179,17 @@ def test_fp32_kernel_variants_agree():
 def test_autotune_keeps_the_state():
     sc = SC.small(mx=40, my=36, p=4, nt=50)
     b1 = SC.build(sc, precision=S.FP32_KAHAN)
+    first = b1.solver.ctx.kernel()
     b1.solver.advance(20, b1.dt)
     best, ms = b1.solver.ctx.autotune(steps=5, reps=2)
     assert best >= 0
     _, r1 = b1.solver.advance(30, b1.dt)
     u1 = b1.solver.ctx.get_state()[0]
+    # the same sequence without the tuning trials in between
     b2 = SC.build(sc, precision=S.FP32_KAHAN)
-    b2.solver.ctx.set_kernel(best)
+    b2.solver.ctx.set_kernel(first)
     b2.solver.advance(20, b2.dt)
+    b2.solver.ctx.set_kernel(best)
     _, r2 = b2.solver.advance(30, b2.dt)
     np.testing.assert_array_equal(r1, r2)
     np.testing.assert_array_equal(u1, b2.solver.ctx.get_state()[0])
@@ -318,3 +321,18 @@ def test_single_station_receivers_sample_the_state(prec):
     e, w, _ = stations[0].get()
     n = el[e][int(np.argmax(w))]
     assert list(b.solver.uv_trace(1)[0][0]) == [u[2 * n], u[2 * n + 1], v[2 * n], v[2 * n + 1]]
+
+
+@pytest.mark.parametrize("p", [3, 8])
+def test_wide_tile_kernel_equals_the_tile_kernel(p):
+    """512 threads per tile instead of 256: the same per-point and per-node
+    arithmetic, so identical bits in every mode (FP64 = the oracle's)."""
+    sc = SC.small(mx=13, my=11, p=p, nt=30, seed=70 + p)
+    for prec in [S.FP64, S.FP32, S.FP32_KAHAN]:
+        _, _, r0, u0, v0, a0 = gpu_run(sc, precision=prec, kernel=S.KERNEL_LDS)
+        for kern in (S.KERNEL_LDS_WIDE,):
+            _, _, r1, u1, v1, a1 = gpu_run(sc, precision=prec, kernel=kern)
+            np.testing.assert_array_equal(r1, r0)
+            np.testing.assert_array_equal(u1, u0)
+            np.testing.assert_array_equal(v1, v0)
+            np.testing.assert_array_equal(a1, a0)
