@@ -27,11 +27,11 @@ __device__ __forceinline__ void ens_loop(const EnsArgs<R> &a, Body body) {
       else hi = mid - 1;
     }
     const int i = __builtin_amdgcn_readfirstlane(lo);
-    StepParams<R> prm = a.prms[2 * i + (a.step & 1)];
-    prm.step = a.step;
-    prm.last = a.last;
     if (!first) __syncthreads();  // the previous tile's last phase has read its LDS
     first = false;
+    StepParams<R> prm = a.prms[2 * i + (a.step & 1)];  // uniform: scalar loads
+    prm.step = a.step;
+    prm.last = a.last;
     body(prm, t - a.tile_off[i]);
   }
 }
