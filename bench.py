@@ -33,6 +33,11 @@ BYTES_PER_NODE = {0: 2 * 8 + 2 * 8 + 8 + 2 * 8 + 2 * 8,   # FP64: 72
 CANONICAL_B_NODE = {0: 152, 1: 76, 2: 92}  # two-pass scheme bytes/node/step
 
 
+def log(msg):
+    """Progress on stderr (the result is the one JSON line on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def rel_l2(a, b):
     n = np.linalg.norm(b)
     return float(np.linalg.norm(a - b) / n) if n > 0 else float(np.linalg.norm(a - b))
@@ -191,6 +196,7 @@ def ensemble_bench(S, SC, n=256, nt=1500):
     builts = [SC.build(sc, precision=S.FP32) for sc in scs]
     ens = S.Ensemble([b.solver for b in builts])
     best, _ = ens.autotune(steps=10, reps=2)
+    log(f"ensemble: {n} members, batched run")
     dts = [b.dt for b in builts]
     t0 = time.perf_counter()
     res = ens.advance(nt, dts)
@@ -201,6 +207,7 @@ def ensemble_bench(S, SC, n=256, nt=1500):
     for b in builts:
         b.solver.close()
     t_seq, errs = 0.0, []
+    log("ensemble: sequential solves")
     for i, sc in enumerate(scs):
         b = SC.build(sc, precision=S.FP32)
         t0 = time.perf_counter()
@@ -225,8 +232,10 @@ def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
     solver and its time per answer against the solver's time per solve."""
     from se2dr_fork_amd import surrogate as SG
 
-    tr = SG.generate(range(ntrain))
-    te = SG.generate(range(100000, 100000 + ntest))
+    log("surrogate: generating the training set")
+    tr = SG.generate(range(ntrain), log=log)
+    te = SG.generate(range(100000, 100000 + ntest), log=log)
+    log("surrogate: training")
     sg = SG.Surrogate()
     losses = sg.fit(tr, epochs=epochs, lr=1e-3)
     train_ms = sg.mlp.timing()[0]
@@ -352,6 +361,7 @@ def main():
     precs = [S.PRECISIONS[x] for x in args.precisions.split(",")]
     res = {}
     for prec in precs:
+        log(f"step kernel, precision {prec}")
         kern = None if (args.kernel == "auto" or prec == S.FP64) else {"lds": S.KERNEL_LDS, "reg": S.KERNEL_REG}[args.kernel]
         res[prec] = run_precision(S, SC, sc, prec, args.steps, args.warmup, args.full_steps, args.profile_steps, kern)
     head = precs[0]
@@ -405,6 +415,7 @@ def main():
     if not args.no_surrogate:
         out["surrogate"] = surrogate_bench(S, SC)
     if not args.no_cpu:
+        log("host CPU baseline")
         out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"])
     else:
         out["cpu_baseline"] = None

@@ -76,8 +76,9 @@ def seismograms(trace: np.ndarray) -> np.ndarray:
     return np.ascontiguousarray(v.transpose(1, 2, 0)).ravel()
 
 
-def solve(scs: Sequence[SC.Scenario], group=1152) -> np.ndarray:
-    """Plain-solver outputs of the scenarios, advanced in ensembles of `group`."""
+def solve(scs: Sequence[SC.Scenario], group=1152, log=None) -> np.ndarray:
+    """Plain-solver outputs of the scenarios, advanced in ensembles of `group`
+    (log(message), if given, is called after each ensemble)."""
     out = np.zeros((len(scs), WIDTHS[-1]))
     for g0 in range(0, len(scs), group):
         builts = [SC.build(sc, precision=FP32) for sc in scs[g0:g0 + group]]
@@ -90,6 +91,8 @@ def solve(scs: Sequence[SC.Scenario], group=1152) -> np.ndarray:
         ens.close()
         for b in builts:
             b.solver.close()
+        if log:
+            log(f"solved {min(g0 + group, len(scs))} of {len(scs)} instances")
     return out
 
 
@@ -100,10 +103,10 @@ class Dataset:
     seconds: float       # wall time of the solves
 
 
-def generate(seeds: Sequence[int], mx=96, p=4, group=1152) -> Dataset:
+def generate(seeds: Sequence[int], mx=96, p=4, group=1152, log=None) -> Dataset:
     pairs = [instance(s, mx, p) for s in seeds]
     t0 = time.perf_counter()
-    y = solve([sc for sc, _ in pairs], group)
+    y = solve([sc for sc, _ in pairs], group, log)
     return Dataset(x=np.array([x for _, x in pairs]), y=y, seconds=time.perf_counter() - t0)
 
 
