@@ -92,9 +92,11 @@ def measured_traffic(mx, p, prec, signature):
     return None
 
 
-# (threads in x, threads in y, minimum waves per SIMD) of the register-blocked
-# kernel's schedules, in the library's variant order (SE2W_KERNEL_REG + k)
-REG_SCHEDULES = [(32, 4, 0), (16, 4, 0), (32, 2, 0), (16, 8, 0), (64, 2, 0), (16, 4, 3), (32, 4, 3), (16, 16, 0)]
+# (elements in x, elements in y, minimum waves per SIMD, node-phase prefetch
+# level, lanes per element) of the register-blocked kernel's schedules, in the
+# library's variant order (SE2W_KERNEL_REG + k)
+REG_SCHEDULES = [(32, 4, 0, 2, 1), (16, 4, 0, 2, 1), (32, 2, 0, 2, 1), (32, 4, 0, 0, 2), (32, 4, 0, 3, 2),
+                 (32, 2, 0, 3, 2), (32, 3, 0, 3, 2), (32, 2, 0, 3, 1)]
 
 
 def kernel_signature(variant, p, mode):
@@ -107,8 +109,8 @@ def kernel_signature(variant, p, mode):
         return f"line_kernel<{p}, {mode}>"
     if variant == 11:
         return f"tile_kernel<{p}, {mode}, true, 512>"
-    ex, ey, w = REG_SCHEDULES[variant - 1]
-    return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}>"
+    ex, ey, w, pf, lpe = REG_SCHEDULES[variant - 1]
+    return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}, {pf}, {lpe}>"
 
 
 def ens_kernel_signature(variant, p, mode):
@@ -121,14 +123,15 @@ def ens_kernel_signature(variant, p, mode):
         return f"ens_line_kernel<{p}, {mode}>"
     if variant == 11:
         return f"ens_tile_kernel<{p}, {mode}, 512>"
-    ex, ey, w = REG_SCHEDULES[variant - 1]
-    return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}>"
+    ex, ey, w, pf, lpe = REG_SCHEDULES[variant - 1]
+    return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}, {pf}, {lpe}>"
 
 
 def kernel_name(variant, p, mode):
     what = ("LDS-staged tile" if variant == 0 else "matrix-core contractions" if variant == 9 else
             "line contractions" if variant == 10 else "LDS-staged tile, 512 threads" if variant == 11 else
-            "thread per element, %dx%d element threads" % REG_SCHEDULES[variant - 1][:2])
+            "%s per element, %dx%d elements" % (("thread", "lane pair")[REG_SCHEDULES[variant - 1][4] - 1],
+                                                *REG_SCHEDULES[variant - 1][:2]))
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
 
 

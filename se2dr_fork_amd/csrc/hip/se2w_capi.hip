@@ -324,8 +324,8 @@ bool launch_reg_s(se2w_ctx *c, const StepParams<float> &prm) {
     if (reg_schedule_built(P, S, FUSED) && reg_schedule_of(c->variant) == S) {
       if constexpr (reg_schedule_built(P, S, FUSED)) {
         constexpr RegSchedule r = kRegSchedules[S];
-        hipLaunchKernelGGL((reg_kernel<P, MODE, FUSED, r.ex, r.ey, r.waves>), dim3(8 * c->tiles_per_xcd),
-                           dim3(r.ex * r.ey), 0, c->stream, prm);
+        hipLaunchKernelGGL((reg_kernel<P, MODE, FUSED, r.ex, r.ey, r.waves, r.pf, r.lpe>), dim3(8 * c->tiles_per_xcd),
+                           dim3(r.ex * r.ey * r.lpe), 0, c->stream, prm);
         return true;
       }
     }

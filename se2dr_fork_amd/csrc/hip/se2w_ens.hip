@@ -29,7 +29,19 @@ __device__ __forceinline__ void ens_loop(const EnsArgs<R> &a, Body body) {
     const int i = __builtin_amdgcn_readfirstlane(lo);
     if (!first) __syncthreads();  // the previous tile's last phase has read its LDS
     first = false;
-    StepParams<R> prm = a.prms[2 * i + (a.step & 1)];  // uniform: scalar loads
+    // The parameter table is read through the constant address space: the
+    // loads are uniform and never clobbered, so they become scalar loads into
+    // SGPRs (a copy through a global pointer would hold the ~60-dword struct in
+    // VGPRs and lower the occupancy).
+    using CWord = const __attribute__((address_space(4))) uint32_t;
+    static_assert(sizeof(StepParams<R>) % 4 == 0, "whole dwords");
+    StepParams<R> prm;
+    {
+      CWord *src = (CWord *)(uintptr_t)(a.prms + 2 * i + (a.step & 1));
+      uint32_t *dst = reinterpret_cast<uint32_t *>(&prm);
+#pragma unroll
+      for (unsigned k = 0; k < sizeof(StepParams<R>) / 4; k++) dst[k] = src[k];
+    }
     prm.step = a.step;
     prm.last = a.last;
     body(prm, t - a.tile_off[i]);
@@ -41,10 +53,10 @@ __global__ __launch_bounds__(NT) void ens_tile_kernel(EnsArgs<typename RealOf<MO
   ens_loop(a, [](const StepParams<typename RealOf<MODE>::T> &prm, int tile) { tile_body<P, MODE, true, NT>(prm, tile); });
 }
 
-template <int P, int MODE, int EX, int EY, int WAVES>
-__global__ __launch_bounds__(EX * EY) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+template <int P, int MODE, int EX, int EY, int WAVES, int PF, int LPE>
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
 void ens_reg_kernel(EnsArgs<float> a) {
-  ens_loop(a, [](const StepParams<float> &prm, int tile) { reg_body<P, MODE, true, EX, EY>(prm, tile); });
+  ens_loop(a, [](const StepParams<float> &prm, int tile) { reg_body<P, MODE, true, EX, EY, PF, LPE>(prm, tile); });
 }
 
 template <int P, int MODE>
@@ -65,15 +77,16 @@ int reg_dispatch(int sched, const void *args, int grid, hipStream_t s, int *bpc,
     if constexpr (reg_schedule_built(P, S, true)) {
       if (sched == S) {
         constexpr RegSchedule r = kRegSchedules[S];
-        auto k = ens_reg_kernel<P, MODE, r.ex, r.ey, r.waves>;
+        auto k = ens_reg_kernel<P, MODE, r.ex, r.ey, r.waves, r.pf, r.lpe>;
+        const int nt = r.ex * r.ey * r.lpe;
         if (!args) {
           int n = 0;
-          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, r.ex * r.ey, 0) != hipSuccess) return -1;
+          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, 0) != hipSuccess) return -1;
           *bpc = n;
-          *thr = r.ex * r.ey;
+          *thr = nt;
           return 0;
         }
-        hipLaunchKernelGGL(k, dim3(grid), dim3(r.ex * r.ey), 0, s, *(const EnsArgs<float> *)args);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(nt), 0, s, *(const EnsArgs<float> *)args);
         return 0;
       }
     }

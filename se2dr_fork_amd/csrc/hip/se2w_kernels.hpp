@@ -101,6 +101,34 @@ __device__ __forceinline__ int tile_of_block(int b, int tiles_per_xcd) {
   return (b & 7) * tiles_per_xcd + (b >> 3);
 }
 
+// The tile's ranges in the per-tile source, receiver and element-receiver
+// lists, read once per tile through the constant address space (the lists are
+// set up before a run and never written by a kernel): scalar loads, so the
+// per-node update carries no memory access for them.
+struct TileLists {
+  int s0, s1;  // sources
+  int r0, r1;  // receivers
+  int q0, q1;  // element receivers
+  __device__ __forceinline__ bool empty() const { return s0 >= s1 && r0 >= r1 && q0 >= q1; }
+};
+
+template <typename R>
+__device__ __forceinline__ TileLists tile_lists(const StepParams<R> &prm, int tile) {
+  using CI = const __attribute__((address_space(4))) int;
+  TileLists t;
+  t.s0 = ((CI *)(uintptr_t)prm.tile_src_ptr)[tile];
+  t.s1 = ((CI *)(uintptr_t)prm.tile_src_ptr)[tile + 1];
+  t.r0 = ((CI *)(uintptr_t)prm.tile_rec_ptr)[tile];
+  t.r1 = ((CI *)(uintptr_t)prm.tile_rec_ptr)[tile + 1];
+  if (prm.ner > 0) {
+    t.q0 = ((CI *)(uintptr_t)prm.tile_erec_ptr)[tile];
+    t.q1 = ((CI *)(uintptr_t)prm.tile_erec_ptr)[tile + 1];
+  } else {
+    t.q0 = t.q1 = 0;
+  }
+  return t;
+}
+
 // ---------------------------------------------------------------------------
 // Phase B: stress at every GLL point of every tile element.
 // ---------------------------------------------------------------------------
@@ -241,11 +269,10 @@ __device__ __forceinline__ void force_f32(const float *Sxx, const float *Syy, co
 __host__ __device__ constexpr int rec_stride(int p) { return 6 + 2 * (p + 1); }
 
 template <int P, typename R>
-__device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int tile, int gx, int gy, R ux, R uy,
-                                                 R vx, R vy, R ax, R ay) {
+__device__ __forceinline__ void record_receivers(const StepParams<R> &prm, const TileLists &tl, int gx, int gy, R ux,
+                                                 R uy, R vx, R vy, R ax, R ay) {
   constexpr int NB = P + 1;
-  const int r0 = prm.tile_rec_ptr[tile], r1 = prm.tile_rec_ptr[tile + 1];
-  for (int t = r0; t < r1; t++) {
+  for (int t = tl.r0; t < tl.r1; t++) {
     const int r = prm.tile_rec_idx[t];
     const int e = prm.rec_elem[r], g = prm.rec_gll[r];
     const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
@@ -257,9 +284,8 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int t
       o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy; o[4] = ax; o[5] = ay;
     }
   }
-  if (prm.ner > 0) {  // element receivers: every node of the element
-    const int q0 = prm.tile_erec_ptr[tile], q1 = prm.tile_erec_ptr[tile + 1];
-    for (int t = q0; t < q1; t++) {
+  {  // element receivers: every node of the element
+    for (int t = tl.q0; t < tl.q1; t++) {
       const int r = prm.tile_erec_idx[t];
       const int e = prm.erec_elem[r];
       const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
@@ -274,12 +300,12 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, int t
 // FP64: a = f / M; v += (0.5 dt) a; record; u += dt v; u += (0.5 dt^2) a;
 // v += (0.5 dt) a -- each operation rounded as in the reference's AXPYs.
 template <int P>
-__device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, int tile, int gx, int gy, long idx,
+__device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, const TileLists &tl, int gx, int gy, long idx,
                                                 double fx, double fy, double ux, double uy, double2 vv, double mass) {
 #pragma clang fp contract(off)
   const double ax = fx / mass, ay = fy / mass;
   const double vx = vv.x + prm.hdt * ax, vy = vv.y + prm.hdt * ay;
-  record_receivers<P, double>(prm, tile, gx, gy, ux, uy, vx, vy, ax, ay);
+  record_receivers<P, double>(prm, tl, gx, gy, ux, uy, vx, vy, ax, ay);
   if (prm.last) {
     prm.u_out[idx] = double2{ux, uy};
     prm.v[idx] = double2{vx, vy};
@@ -296,12 +322,12 @@ __device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, i
 // FP32: same sequence with fused multiply-adds; MODE_F32K accumulates the
 // displacement with a compensated (Kahan) sum, one per reference AXPY.
 template <int P, int MODE>
-__device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, int tile, int gx, int gy, long idx,
+__device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, const TileLists &tl, int gx, int gy, long idx,
                                                 float fx, float fy, float ux, float uy, float2 vv, float minv,
                                                 float2 cc) {
   const float ax = fx * minv, ay = fy * minv;
   const float vx = fmaf(prm.hdt, ax, vv.x), vy = fmaf(prm.hdt, ay, vv.y);
-  record_receivers<P, float>(prm, tile, gx, gy, ux, uy, vx, vy, ax, ay);
+  record_receivers<P, float>(prm, tl, gx, gy, ux, uy, vx, vy, ax, ay);
   if (prm.last) {
     prm.u_out[idx] = float2{ux, uy};
     prm.v[idx] = float2{vx, vy};
@@ -331,8 +357,8 @@ __device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, in
 // Node update with the nodal v, mass and compensation already loaded.
 // ---------------------------------------------------------------------------
 template <int P, int MODE>
-__device__ __forceinline__ void node_update_core(const StepParams<typename RealOf<MODE>::T> &prm, int tile, int gx,
-                                                 int gy, long idx, typename RealOf<MODE>::T fx,
+__device__ __forceinline__ void node_update_core(const StepParams<typename RealOf<MODE>::T> &prm, const TileLists &tl,
+                                                 int gx, int gy, long idx, typename RealOf<MODE>::T fx,
                                                  typename RealOf<MODE>::T fy, typename RealOf<MODE>::T ux,
                                                  typename RealOf<MODE>::T uy, typename V2<typename RealOf<MODE>::T>::T vv,
                                                  typename RealOf<MODE>::T mass,
@@ -342,11 +368,10 @@ __device__ __forceinline__ void node_update_core(const StepParams<typename RealO
   constexpr int NB = P + 1;
   // g = sum_s psi_s * vals_s in source order, then f = f + g (3476-3495, 4739)
   {
-    const int s0 = prm.tile_src_ptr[tile], s1 = prm.tile_src_ptr[tile + 1];
-    if (s0 < s1) {
+    if (tl.s0 < tl.s1) {
       R gxv = R(0), gyv = R(0);
       bool hit = false;
-      for (int t = s0; t < s1; t++) {
+      for (int t = tl.s0; t < tl.s1; t++) {
         const int s = prm.tile_src_idx[t];
         const int e = prm.src_elem[s];
         const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
@@ -365,9 +390,9 @@ __device__ __forceinline__ void node_update_core(const StepParams<typename RealO
     }
   }
   if constexpr (MODE == MODE_F64) {
-    node_finish_f64<P>(prm, tile, gx, gy, idx, fx, fy, ux, uy, vv, mass);
+    node_finish_f64<P>(prm, tl, gx, gy, idx, fx, fy, ux, uy, vv, mass);
   } else {
-    node_finish_f32<P, MODE>(prm, tile, gx, gy, idx, fx, fy, ux, uy, vv, mass, cc);
+    node_finish_f32<P, MODE>(prm, tl, gx, gy, idx, fx, fy, ux, uy, vv, mass, cc);
   }
 }
 
@@ -376,15 +401,15 @@ __device__ __forceinline__ void node_update_core(const StepParams<typename RealO
 //   f = f + g;  a = f / M;  v = v + (0.5 dt) a;  [record u, v, a]
 //   u = u + dt v;  u = u + (0.5 dt^2) a;  v = v + (0.5 dt) a.
 template <int P, int MODE>
-__device__ __forceinline__ void node_update(const StepParams<typename RealOf<MODE>::T> &prm, int tile, int gx,
-                                            int gy, typename RealOf<MODE>::T fx, typename RealOf<MODE>::T fy,
+__device__ __forceinline__ void node_update(const StepParams<typename RealOf<MODE>::T> &prm, const TileLists &tl,
+                                            int gx, int gy, typename RealOf<MODE>::T fx, typename RealOf<MODE>::T fy,
                                             typename RealOf<MODE>::T ux, typename RealOf<MODE>::T uy) {
   using R = typename RealOf<MODE>::T;
   using R2 = typename V2<R>::T;
   const long idx = (long)gy * prm.pitch + gx;
   R2 cc = R2{R(0), R(0)};
   if constexpr (MODE == MODE_F32K) cc = prm.c_in[idx];
-  node_update_core<P, MODE>(prm, tile, gx, gy, idx, fx, fy, ux, uy, prm.v[idx], prm.mass[idx], cc);
+  node_update_core<P, MODE>(prm, tl, gx, gy, idx, fx, fy, ux, uy, prm.v[idx], prm.mass[idx], cc);
 }
 
 // ---------------------------------------------------------------------------
@@ -469,6 +494,7 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   }
   __syncthreads();
   // Phase C (+D): owned nodes.
+  const TileLists tl = tile_lists(prm, tile);
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
@@ -513,7 +539,7 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
     }
     if constexpr (FUSED) {
       const int lx = gx - gx0, ly = gy - gy0;
-      node_update<P, MODE>(prm, tile, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
+      node_update<P, MODE>(prm, tl, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
     } else {
       prm.f[(long)gy * prm.pitch + gx] = R2{fx, fy};
     }
@@ -542,13 +568,14 @@ __global__ __launch_bounds__(kThreads) void update_kernel(StepParams<typename Re
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
+  const TileLists tl = tile_lists(prm, tile);
   for (int k = threadIdx.x; k < OX * OY; k += kThreads) {
     const int oy = k / OX, ox = k - oy * OX;
     const int gx = ex0 * P + ox, gy = ey0 * P + oy;
     const long idx = (long)gy * prm.pitch + gx;
     const R2 f = prm.f[idx];
     const R2 u = prm.u_in[idx];
-    node_update<P, MODE>(prm, tile, gx, gy, f.x, f.y, u.x, u.y);
+    node_update<P, MODE>(prm, tl, gx, gy, f.x, f.y, u.x, u.y);
   }
 }
 

@@ -309,6 +309,7 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
   }
   __syncthreads();
   // D: owned nodes: element contributions in ascending element order, update.
+  const TileLists tl = tile_lists(prm, tile);
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
@@ -342,7 +343,7 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
       }
     }
     const int lx = gx - gx0, ly = gy - gy0;
-    node_update<P, MODE>(prm, tile, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
+    node_update<P, MODE>(prm, tl, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
   }
 }
 

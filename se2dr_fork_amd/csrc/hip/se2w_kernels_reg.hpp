@@ -1,9 +1,16 @@
 // se2w_kernels_reg.hpp -- register-blocked FP32 step kernel.
 //
-// One thread per element.  The element's (p+1)^2 displacements are loaded
-// from HBM/L2 into registers; the tensor-product contractions run row by row
-// with the derivative matrix and quadrature weights in scalar registers (the
-// same for every lane), and the nodal forces accumulate in registers.
+// One element per thread (LPE = 1) or per pair of adjacent lanes (LPE = 2).
+// The element's (p+1)^2 displacements are loaded from HBM/L2 into registers;
+// the tensor-product contractions run row by row with the derivative matrix
+// and quadrature weights in scalar registers (the same for every lane), and
+// the nodal forces accumulate in registers.  With LPE = 2 each lane of a pair
+// holds one displacement component and accumulates one force component: the
+// lane computes the x and y derivatives of its component, swaps them with its
+// partner (one DPP move each) and forms the two stress combinations its force
+// component needs, so the pair does the work of one LPE = 1 thread with half
+// the registers per lane.  Both forms perform the same floating-point
+// operations in the same order and give identical bits.
 //
 // Node ownership: element (ex, ey) owns its nodes with local i < p and j < p
 // (plus i = p on the domain's right boundary and j = p on its top boundary),
@@ -15,20 +22,22 @@
 // node row, one node per lane) applies the Verlet update with coalesced
 // global loads and stores.
 //
-// Workgroup = EX x EY element threads: (EX-1) x (EY-1) owned elements plus
-// the halo column to the left and the halo row below, which only publish
-// their edges.
+// Workgroup = EX x EY elements: (EX-1) x (EY-1) owned elements plus the halo
+// column to the left and the halo row below, which only publish their edges.
 #pragma once
 #include "se2w_kernels.hpp"
 
 namespace se2w {
 
-// Schedules of the register-blocked kernel: workgroup shape (element threads
-// EX x EY) and a minimum number of waves per SIMD (0 = compiler's choice),
-// which caps the registers per lane.  The autotuner picks one per problem.
-struct RegSchedule { int ex, ey, waves; };
-constexpr RegSchedule kRegSchedules[] = {{32, 4, 0}, {16, 4, 0}, {32, 2, 0}, {16, 8, 0}, {64, 2, 0}, {16, 4, 3},
-                                         {32, 4, 3}, {16, 16, 0}};
+// Schedules of the register-blocked kernel: workgroup shape (EX x EY
+// elements), a minimum number of waves per SIMD (0 = compiler's choice, which
+// caps the registers per lane), the node-phase loads issued before the force
+// exchange (pf 2: u, v, M and the compensation; 1: u, v, M; 0: none, all
+// loaded in the node phase) and the lanes per element.  The autotuner picks
+// one per problem.
+struct RegSchedule { int ex, ey, waves, pf, lpe; };
+constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
+                                         {32, 4, 0, 3, 2}, {32, 2, 0, 3, 2}, {32, 3, 0, 3, 2}, {32, 2, 0, 3, 1}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
 // Schedules instantiated for (p, fused): all of them for p = 4, the first
@@ -36,14 +45,24 @@ constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]
 // element kernel.
 constexpr bool reg_schedule_built(int P, int S, bool fused) { return fused ? (P == 4 || S < 3) : S == 0; }
 
-// One workgroup of EX x EY threads processes one tile of (EX-1) x (EY-1)
-// elements (all threads call it with the same tile).  A workgroup that calls
-// it for several tiles in a row puts a barrier between the calls.
-template <int P, int MODE, bool FUSED, int EX, int EY>
+// Partner lane's value (lanes 2k and 2k+1 swap): DPP quad_perm [1, 0, 3, 2].
+__device__ __forceinline__ float pair_swap(float x) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(x), 0xB1, 0xF, 0xF, false));
+}
+
+// Component c (0 = x, 1 = y) of an LDS float2.
+__device__ __forceinline__ float &comp(float2 &v, int c) { return reinterpret_cast<float *>(&v)[c]; }
+
+// One workgroup of EX x EY x LPE threads processes one tile of (EX-1) x
+// (EY-1) elements (all threads call it with the same tile).  A workgroup that
+// calls it for several tiles in a row puts a barrier between the calls.
+template <int P, int MODE, bool FUSED, int EX, int EY, int PF, int LPE>
 __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile) {
   static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
+  static_assert(LPE == 1 || LPE == 2, "one or two lanes per element");
   constexpr int NB = P + 1;
   constexpr int TEX = EX - 1, TEY = EY - 1;
+  constexpr int NT = EX * EY * LPE;
   __shared__ float2 redge[EY][EX][NB];  // right edge (i = p), j = 0..p
   __shared__ float2 tedge[EY][EX][NB];  // top edge (j = p), i = 0..p
   // nodal forces of the owned nodes; column I stored at I + I/P
@@ -52,7 +71,9 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
   const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
   const int ex0 = tx * TEX, ey0 = ty * TEY;
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
-  const int lx = threadIdx.x % EX, ly = threadIdx.x / EX;
+  const int c = LPE == 2 ? (int)(threadIdx.x & 1) : 0;  // component of this lane (LPE = 2)
+  const int el = threadIdx.x / LPE;
+  const int lx = el % EX, ly = el / EX;
   const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
   const bool valid = ex >= 0 && ey >= 0 && ex < ex_end && ey < ey_end;
   const bool owner = valid && lx >= 1 && ly >= 1;
@@ -60,27 +81,69 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
   // node row, one node per lane
   const int ox1 = ex_end * P - 1 + (ex_end == prm.mx ? 1 : 0), oy1 = ey_end * P - 1 + (ey_end == prm.my ? 1 : 0);
   const int OX = ox1 - ex0 * P + 1, OY = oy1 - ey0 * P + 1;
-  constexpr int LANES = EX * EY < 64 ? EX * EY : 64;         // threads per node row
-  constexpr int NWAVES = (EX * EY) / LANES;                   // node rows in flight
-  static_assert(NWAVES * LANES == EX * EY, "workgroup must be a multiple of its row width");
+  constexpr int LANES = NT < 64 ? NT : 64;                   // threads per node row
+  constexpr int NWAVES = NT / LANES;                          // node rows in flight
+  static_assert(NWAVES * LANES == NT, "workgroup must be a multiple of its row width");
   constexpr int RPW = (TEY * P + 1 + NWAVES - 1) / NWAVES;  // node rows per row group (max)
   constexpr int CPL = (TEX * P + 1 + LANES - 1) / LANES;    // node columns per thread (max)
   const int wave = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  // The halo row below the tile fills whole waves when a row of elements is
+  // a multiple of 64 lanes: those waves take a shorter path (top edge only).
+  constexpr bool HALO_WAVES = (EX * LPE) % 64 == 0;
 
+  // The node phase's loads do not depend on the forces.  All of a thread's
+  // node loads are issued together, ahead of the updates: pf 3 before the
+  // element's loads, pf 2 after the element operator (their latency overlaps
+  // the force exchange), pf 0 after the exchange barrier (overlapping the
+  // owners' sums); pf 1 splits u, v, M (as pf 2) from the compensation (as
+  // pf 0).
+  float2 pu[RPW][CPL], pv[RPW][CPL], pc[RPW][CPL];
+  float pm[RPW][CPL];
+  // Slots past the tile's owned block load its last owned node instead (in
+  // bounds, unused): no branch between the loads, so they all issue before
+  // the first wait.
+  auto node_loads = [&](int r, int cidx, bool uvm, bool compensation) {
+    const int lyn = min(wave + r * NWAVES, OY - 1), lxn = min(lane + cidx * LANES, OX - 1);
+    const long idx = (long)(ey0 * P + lyn) * prm.pitch + ex0 * P + lxn;
+    if (uvm) {
+      pu[r][cidx] = prm.u_in[idx];
+      pv[r][cidx] = prm.v[idx];
+      pm[r][cidx] = prm.mass[idx];
+    }
+    if (compensation) {
+      if constexpr (MODE == MODE_F32K) pc[r][cidx] = prm.c_in[idx];
+      else pc[r][cidx] = float2{0.f, 0.f};
+    }
+  };
+  if constexpr (FUSED && PF == 3) {
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int cidx = 0; cidx < CPL; cidx++) node_loads(r, cidx, true, true);
+  }
+  // LPE = 1: ux, uy, fx, fy hold both components; LPE = 2: ux and fx hold
+  // this lane's component (uy, fy unused).
   float ux[NB][NB], uy[NB][NB];  // [j][i]
   float fx[NB][NB], fy[NB][NB];
-  long base = 0;
   if (valid) {
-    base = (long)(ey * P) * prm.pitch + ex * P;
-    const float2 *ub = prm.u_in + base;
+    const long base = (long)(ey * P) * prm.pitch + ex * P;
+    if constexpr (LPE == 1) {
+      const float2 *ub = prm.u_in + base;
 #pragma unroll
-    for (int j = 0; j < NB; j++) {
+      for (int j = 0; j < NB; j++) {
 #pragma unroll
-      for (int i = 0; i < NB; i++) {
-        const float2 t = ub[(long)j * prm.pitch + i];
-        ux[j][i] = t.x;
-        uy[j][i] = t.y;
+        for (int i = 0; i < NB; i++) {
+          const float2 t = ub[(long)j * prm.pitch + i];
+          ux[j][i] = t.x;
+          uy[j][i] = t.y;
+        }
       }
+    } else {
+      const float *ub = reinterpret_cast<const float *>(prm.u_in + base) + c;
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) ux[j][i] = ub[2 * ((long)j * prm.pitch + i)];
     }
     const int e = ex + ey * prm.mx;
     const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
@@ -91,15 +154,23 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
     const float s = -detJ;
     const float a1 = s * c11 * iJ00 * iJ00, a2 = s * lam * iJ00 * iJ11, a3 = s * c11 * iJ11 * iJ11;
     const float b1 = s * mu * iJ11 * iJ11, b2 = s * mu * iJ00 * iJ11, b3 = s * mu * iJ00 * iJ00;
-    const float *__restrict__ D = prm.D;
-    const float *__restrict__ W = prm.w2;
-#pragma unroll
-    for (int j = 0; j < NB; j++)
-#pragma unroll
-      for (int i = 0; i < NB; i++) fx[j][i] = fy[j][i] = 0.f;
-#pragma unroll
-    for (int qj = 0; qj < NB; qj++) {
-      float A[NB], B[NB], Cx[NB], E[NB];
+    // D and the weights are read through the constant address space: uniform,
+    // never written by a kernel, so they are scalar loads into SGPRs also when
+    // prm itself is not a kernel argument (the batched kernels).
+    using CF = const __attribute__((address_space(4))) float;
+    CF *D = (CF *)(uintptr_t)prm.D;
+    CF *W = (CF *)(uintptr_t)prm.w2;
+    // Fac-weighted stresses of quadrature row qj, pre-multiplied by the
+    // Jacobian factor of the test-function derivative they meet:
+    //   A  = w (a1 dx ux + a2 dy uy)  iJ00 fac sxx  (row terms of fx)
+    //   B  = w (b1 dy ux + b2 dx uy)  iJ11 fac sxy  (column terms of fx)
+    //   Cx = w (b3 dx uy + b2 dy ux)  iJ00 fac sxy  (row terms of fy)
+    //   E  = w (a3 dy uy + a2 dx ux)  iJ11 fac syy  (column terms of fy)
+    // The first product of each is the one of the lane's own component, so
+    // LPE = 2 forms R (A or Cx) and Cq (B or E) from its own derivatives and
+    // the partner's with the same operations: R = w (rA own_dx + rB other_dy),
+    // Cq = w (cA own_dy + cB other_dx).
+    auto row_stress = [&](int qj, float *A, float *B, float *Cx, float *E) {
 #pragma unroll
       for (int qi = 0; qi < NB; qi++) {
         float dxux = 0.f, dxuy = 0.f, dyux = 0.f, dyuy = 0.f;
@@ -111,59 +182,121 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
           dyuy = fmaf(D[qj * NB + k], uy[k][qi], dyuy);
         }
         const float w = W[qi + qj * NB];
-        A[qi] = w * fmaf(a1, dxux, a2 * dyuy);   // iJ00 * fac * sxx
-        E[qi] = w * fmaf(a2, dxux, a3 * dyuy);   // iJ11 * fac * syy
-        B[qi] = w * fmaf(b1, dyux, b2 * dxuy);   // iJ11 * fac * sxy
-        Cx[qi] = w * fmaf(b2, dyux, b3 * dxuy);  // iJ00 * fac * sxy
+        A[qi] = w * fmaf(a1, dxux, a2 * dyuy);
+        E[qi] = w * fmaf(a3, dyuy, a2 * dxux);
+        B[qi] = w * fmaf(b1, dyux, b2 * dxuy);
+        Cx[qi] = w * fmaf(b3, dxuy, b2 * dyux);
       }
-      // row terms: nodes (i, qj) from the points of row qj
+    };
+    const float rA = c ? b3 : a1, rB = c ? b2 : a2, cA = c ? a3 : b1, cB = c ? a2 : b2;
+    auto row_stress_pair = [&](int qj, float *R, float *Cq) {
 #pragma unroll
-      for (int i = 0; i < NB; i++) {
+      for (int qi = 0; qi < NB; qi++) {
+        float dx = 0.f, dy = 0.f;
+#pragma unroll
+        for (int k = 0; k < NB; k++) {
+          dx = fmaf(D[qi * NB + k], ux[qj][k], dx);
+          dy = fmaf(D[qj * NB + k], ux[k][qi], dy);
+        }
+        const float odx = pair_swap(dx), ody = pair_swap(dy);
+        const float w = W[qi + qj * NB];
+        R[qi] = w * fmaf(rA, dx, rB * ody);
+        Cq[qi] = w * fmaf(cA, dy, cB * odx);
+      }
+    };
+    if (HALO_WAVES && ly == 0) {
+      // Halo row: only the top edge (j = p; node (p, p) is also the corner
+      // the owners read from redge), with the same operations in the same
+      // order as the full path below, so the edge values are identical to it.
+      float tx[NB], ty[NB];
+#pragma unroll
+      for (int i = 0; i < NB; i++) tx[i] = ty[i] = 0.f;
+#pragma unroll
+      for (int qj = 0; qj < NB; qj++) {
+        float A[NB], B[NB], Cx[NB], E[NB];
+        if constexpr (LPE == 1) row_stress(qj, A, B, Cx, E);
+        else row_stress_pair(qj, A, B);
+        if (qj == P) {
+#pragma unroll
+          for (int i = 0; i < NB; i++) {
+#pragma unroll
+            for (int qi = 0; qi < NB; qi++) {
+              tx[i] = fmaf(D[qi * NB + i], A[qi], tx[i]);
+              if constexpr (LPE == 1) ty[i] = fmaf(D[qi * NB + i], Cx[qi], ty[i]);
+            }
+          }
+        }
 #pragma unroll
         for (int qi = 0; qi < NB; qi++) {
-          fx[qj][i] = fmaf(D[qi * NB + i], A[qi], fx[qj][i]);
-          fy[qj][i] = fmaf(D[qi * NB + i], Cx[qi], fy[qj][i]);
+          tx[qi] = fmaf(D[qj * NB + P], B[qi], tx[qi]);
+          if constexpr (LPE == 1) ty[qi] = fmaf(D[qj * NB + P], E[qi], ty[qi]);
         }
       }
-      // column terms: nodes (qi, j) from the point (qi, qj)
+      if constexpr (LPE == 1) {
 #pragma unroll
-      for (int j = 0; j < NB; j++) {
+        for (int k = 0; k < NB; k++) tedge[ly][lx][k] = float2{tx[k], ty[k]};
+        redge[ly][lx][P] = float2{tx[P], ty[P]};
+      } else {
 #pragma unroll
-        for (int qi = 0; qi < NB; qi++) {
-          fx[j][qi] = fmaf(D[qj * NB + j], B[qi], fx[j][qi]);
-          fy[j][qi] = fmaf(D[qj * NB + j], E[qi], fy[j][qi]);
+        for (int k = 0; k < NB; k++) comp(tedge[ly][lx][k], c) = tx[k];
+        comp(redge[ly][lx][P], c) = tx[P];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) fx[j][i] = fy[j][i] = 0.f;
+#pragma unroll
+      for (int qj = 0; qj < NB; qj++) {
+        float A[NB], B[NB], Cx[NB], E[NB];
+        if constexpr (LPE == 1) row_stress(qj, A, B, Cx, E);
+        else row_stress_pair(qj, A, B);
+        // row terms: nodes (i, qj) from the points of row qj
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+#pragma unroll
+          for (int qi = 0; qi < NB; qi++) {
+            fx[qj][i] = fmaf(D[qi * NB + i], A[qi], fx[qj][i]);
+            if constexpr (LPE == 1) fy[qj][i] = fmaf(D[qi * NB + i], Cx[qi], fy[qj][i]);
+          }
+        }
+        // column terms: nodes (qi, j) from the point (qi, qj)
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+#pragma unroll
+          for (int qi = 0; qi < NB; qi++) {
+            fx[j][qi] = fmaf(D[qj * NB + j], B[qi], fx[j][qi]);
+            if constexpr (LPE == 1) fy[j][qi] = fmaf(D[qj * NB + j], E[qi], fy[j][qi]);
+          }
         }
       }
-    }
-    // publish the right and top edges for the neighbouring owners
+      // publish the right and top edges for the neighbouring owners
 #pragma unroll
-    for (int k = 0; k < NB; k++) {
-      redge[ly][lx][k] = float2{fx[k][P], fy[k][P]};
-      tedge[ly][lx][k] = float2{fx[P][k], fy[P][k]};
+      for (int k = 0; k < NB; k++) {
+        if constexpr (LPE == 1) {
+          redge[ly][lx][k] = float2{fx[k][P], fy[k][P]};
+          tedge[ly][lx][k] = float2{fx[P][k], fy[P][k]};
+        } else {
+          comp(redge[ly][lx][k], c) = fx[k][P];
+          comp(tedge[ly][lx][k], c) = fx[P][k];
+        }
+      }
     }
   }
-  // The node phase's loads do not depend on the forces: issue them all now so
-  // that their latency overlaps the force exchange.
-  float2 pu[RPW][CPL], pv[RPW][CPL], pc[RPW][CPL];
-  float pm[RPW][CPL];
-  if constexpr (FUSED) {
+  if constexpr (FUSED && (PF == 1 || PF == 2)) {
 #pragma unroll
-    for (int r = 0; r < RPW; r++) {
+    for (int r = 0; r < RPW; r++)
 #pragma unroll
-      for (int cidx = 0; cidx < CPL; cidx++) {
-        const int lyn = wave + r * NWAVES, lxn = lane + cidx * LANES;
-        if (lyn < OY && lxn < OX) {
-          const long idx = (long)(ey0 * P + lyn) * prm.pitch + ex0 * P + lxn;
-          pu[r][cidx] = prm.u_in[idx];
-          pv[r][cidx] = prm.v[idx];
-          pm[r][cidx] = prm.mass[idx];
-          if constexpr (MODE == MODE_F32K) pc[r][cidx] = prm.c_in[idx];
-          else pc[r][cidx] = float2{0.f, 0.f};
-        }
-      }
-    }
+      for (int cidx = 0; cidx < CPL; cidx++) node_loads(r, cidx, true, PF == 2);
   }
   __syncthreads();
+  const TileLists tl = tile_lists(prm, tile);
+  if constexpr (FUSED && PF < 2) {
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int cidx = 0; cidx < CPL; cidx++) node_loads(r, cidx, PF == 0, true);
+  }
   // Owners combine their own and the neighbours' contributions in ascending
   // element order and store the nodal force in a node-major LDS tile.
   if (owner) {
@@ -174,22 +307,34 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
 #pragma unroll
       for (int i = 0; i < NB; i++) {
         if ((i < P || rb) && (j < P || tb)) {
-          float gx = fx[j][i], gy = fy[j][i];
-          if (i == 0 && j == 0 && left && below) {  // (ex-1,ey-1), (ex,ey-1), (ex-1,ey), own
-            const float2 d = redge[ly - 1][lx - 1][P], b = tedge[ly - 1][lx][0], l = redge[ly][lx - 1][0];
-            gx = ((d.x + b.x) + l.x) + gx;
-            gy = ((d.y + b.y) + l.y) + gy;
-          } else if (i == 0 && left) {
-            const float2 l = redge[ly][lx - 1][j];
-            gx = l.x + gx;
-            gy = l.y + gy;
-          } else if (j == 0 && below) {
-            const float2 b = tedge[ly - 1][lx][i];
-            gx = b.x + gx;
-            gy = b.y + gy;
-          }
           const int lxn = (lx - 1) * P + i, lyn = (ly - 1) * P + j;
-          sf[lyn][lxn + lxn / P] = float2{gx, gy};
+          if constexpr (LPE == 1) {
+            float gx = fx[j][i], gy = fy[j][i];
+            if (i == 0 && j == 0 && left && below) {  // (ex-1,ey-1), (ex,ey-1), (ex-1,ey), own
+              const float2 d = redge[ly - 1][lx - 1][P], b = tedge[ly - 1][lx][0], l = redge[ly][lx - 1][0];
+              gx = ((d.x + b.x) + l.x) + gx;
+              gy = ((d.y + b.y) + l.y) + gy;
+            } else if (i == 0 && left) {
+              const float2 l = redge[ly][lx - 1][j];
+              gx = l.x + gx;
+              gy = l.y + gy;
+            } else if (j == 0 && below) {
+              const float2 b = tedge[ly - 1][lx][i];
+              gx = b.x + gx;
+              gy = b.y + gy;
+            }
+            sf[lyn][lxn + lxn / P] = float2{gx, gy};
+          } else {
+            float g = fx[j][i];
+            if (i == 0 && j == 0 && left && below) {
+              g = ((comp(redge[ly - 1][lx - 1][P], c) + comp(tedge[ly - 1][lx][0], c)) + comp(redge[ly][lx - 1][0], c)) + g;
+            } else if (i == 0 && left) {
+              g = comp(redge[ly][lx - 1][j], c) + g;
+            } else if (j == 0 && below) {
+              g = comp(tedge[ly - 1][lx][i], c) + g;
+            }
+            comp(sf[lyn][lxn + lxn / P], c) = g;
+          }
         }
       }
     }
@@ -206,7 +351,7 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
         const long idx = (long)gy * prm.pitch + gx;
         const float2 f = sf[lyn][lxn + lxn / P];
         if constexpr (FUSED) {
-          node_update_core<P, MODE>(prm, tile, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
+          node_update_core<P, MODE>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
                                     pm[r][cidx], pc[r][cidx]);
         } else {
           prm.f[idx] = f;
@@ -216,12 +361,12 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
   }
 }
 
-template <int P, int MODE, bool FUSED, int EX, int EY, int WAVES>
-__global__ __launch_bounds__(EX * EY) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+template <int P, int MODE, bool FUSED, int EX, int EY, int WAVES, int PF, int LPE>
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
 void reg_kernel(StepParams<float> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  reg_body<P, MODE, FUSED, EX, EY>(prm, tile);
+  reg_body<P, MODE, FUSED, EX, EY, PF, LPE>(prm, tile);
 }
 
 }  // namespace se2w
