@@ -456,6 +456,28 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
       econ[4][k] = prm.mu[ex + ey * prm.mx];
     }
   }
+  // Node data of the update (v, M, compensation) for the thread's owned
+  // nodes, loaded now so that the latency overlaps phases A and B.  Slots past
+  // the owned block load its last node (in bounds, unused), so the loads carry
+  // no branches and all issue before the first wait.
+  const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
+  const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
+  const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
+  constexpr int NSLOT = ((TEX * P + 1) * (TEY * P + 1) + NT - 1) / NT;  // owned nodes per thread (max)
+  R2 pv[NSLOT], pc[NSLOT];
+  R pm[NSLOT];
+  if constexpr (FUSED) {
+#pragma unroll
+    for (int sl = 0; sl < NSLOT; sl++) {
+      const int k = min(tid + sl * NT, OX * OY - 1);
+      const int oy = k / OX, ox = k - oy * OX;
+      const long idx = (long)(ey0 * P + oy) * prm.pitch + ex0 * P + ox;
+      pv[sl] = prm.v[idx];
+      pm[sl] = prm.mass[idx];
+      if constexpr (MODE == MODE_F32K) pc[sl] = prm.c_in[idx];
+      else pc[sl] = R2{R(0), R(0)};
+    }
+  }
   // Phase A: displacement of the tile's nodes (+ halo) into LDS.
   for (int k = tid; k < NXL * NYL; k += NT) {
     const int ly = k / NXL, lx = k - ly * NXL;
@@ -495,10 +517,10 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   __syncthreads();
   // Phase C (+D): owned nodes.
   const TileLists tl = tile_lists(prm, tile);
-  const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
-  const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
-  const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
-  for (int k = tid; k < OX * OY; k += NT) {
+#pragma unroll
+  for (int sl = 0; sl < NSLOT; sl++) {
+    const int k = tid + sl * NT;
+    if (k >= OX * OY) break;
     const int oy = k / OX, ox = k - oy * OX;
     const int gx = ex0 * P + ox, gy = ey0 * P + oy;
     int exs[2], eys[2], nex = 0, ney = 0;
@@ -539,7 +561,8 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
     }
     if constexpr (FUSED) {
       const int lx = gx - gx0, ly = gy - gy0;
-      node_update<P, MODE>(prm, tl, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
+      const long idx = (long)gy * prm.pitch + gx;
+      node_update_core<P, MODE>(prm, tl, gx, gy, idx, fx, fy, su[0][ly][lx], su[1][ly][lx], pv[sl], pm[sl], pc[sl]);
     } else {
       prm.f[(long)gy * prm.pitch + gx] = R2{fx, fy};
     }

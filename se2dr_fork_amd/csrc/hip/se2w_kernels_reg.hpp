@@ -37,13 +37,13 @@ namespace se2w {
 // one per problem.
 struct RegSchedule { int ex, ey, waves, pf, lpe; };
 constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
-                                         {32, 4, 0, 3, 2}, {32, 2, 0, 3, 2}, {32, 3, 0, 3, 2}, {32, 2, 0, 3, 1}};
+                                         {16, 4, 0, 0, 1}, {16, 8, 0, 0, 2}, {32, 3, 0, 0, 2}, {16, 4, 0, 0, 2}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
 // Schedules instantiated for (p, fused): all of them for p = 4, the first
-// three for the other degrees, the first one for the split schedule's
-// element kernel.
-constexpr bool reg_schedule_built(int P, int S, bool fused) { return fused ? (P == 4 || S < 3) : S == 0; }
+// four for the other degrees, the first one for the split schedule's element
+// kernel.
+constexpr bool reg_schedule_built(int P, int S, bool fused) { return fused ? (P == 4 || S < 4) : S == 0; }
 
 // Partner lane's value (lanes 2k and 2k+1 swap): DPP quad_perm [1, 0, 3, 2].
 __device__ __forceinline__ float pair_swap(float x) {

@@ -156,11 +156,13 @@ def test_assemble_forces(oracle):
                 assert rel_l2(f, fo) < 2e-6
 
 
-def test_fp32_kernel_variants_agree():
-    """Register-blocked schedules share the arithmetic: identical bits.  The
+@pytest.mark.parametrize("p, mx, my", [(4, 37, 33), (6, 23, 19)])
+def test_fp32_kernel_variants_agree(p, mx, my):
+    """Register-blocked schedules share the arithmetic (one lane or a lane pair
+    per element, full or top-edge-only halo rows): identical bits.  The
     LDS-staged kernel sums in another order: agreement to FP32 rounding."""
     hip, _ = S.libraries()
-    sc = SC.small(mx=37, my=33, p=4, nt=60)
+    sc = SC.small(mx=mx, my=my, p=p, nt=60)
     for prec in [S.FP32, S.FP32_KAHAN]:
         _, _, r0, u0, _, _ = gpu_run(sc, precision=prec, kernel=S.KERNEL_LDS)
         ref = None
