@@ -457,16 +457,19 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
     }
   }
   // Node data of the update (v, M, compensation) for the thread's owned
-  // nodes, loaded now so that the latency overlaps phases A and B.  Slots past
-  // the owned block load its last node (in bounds, unused), so the loads carry
-  // no branches and all issue before the first wait.
+  // nodes.  For p <= 6 all of them are loaded at once before phase A, so that
+  // the latency overlaps phases A and B; slots past the owned block load its
+  // last node (in bounds, unused), so the loads carry no branches and all
+  // issue before the first wait.  For p >= 7 the registers would lower the
+  // occupancy: each node's data is loaded in phase D.
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
   constexpr int NSLOT = ((TEX * P + 1) * (TEY * P + 1) + NT - 1) / NT;  // owned nodes per thread (max)
+  constexpr bool EARLY = P <= 6;
   R2 pv[NSLOT], pc[NSLOT];
   R pm[NSLOT];
-  if constexpr (FUSED) {
+  auto node_loads = [&]() {
 #pragma unroll
     for (int sl = 0; sl < NSLOT; sl++) {
       const int k = min(tid + sl * NT, OX * OY - 1);
@@ -477,7 +480,9 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
       if constexpr (MODE == MODE_F32K) pc[sl] = prm.c_in[idx];
       else pc[sl] = R2{R(0), R(0)};
     }
-  }
+  };
+  if constexpr (FUSED && EARLY) node_loads();
+  (void)node_loads;
   // Phase A: displacement of the tile's nodes (+ halo) into LDS.
   for (int k = tid; k < NXL * NYL; k += NT) {
     const int ly = k / NXL, lx = k - ly * NXL;
@@ -561,8 +566,12 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
     }
     if constexpr (FUSED) {
       const int lx = gx - gx0, ly = gy - gy0;
-      const long idx = (long)gy * prm.pitch + gx;
-      node_update_core<P, MODE>(prm, tl, gx, gy, idx, fx, fy, su[0][ly][lx], su[1][ly][lx], pv[sl], pm[sl], pc[sl]);
+      if constexpr (EARLY) {
+        const long idx = (long)gy * prm.pitch + gx;
+        node_update_core<P, MODE>(prm, tl, gx, gy, idx, fx, fy, su[0][ly][lx], su[1][ly][lx], pv[sl], pm[sl], pc[sl]);
+      } else {
+        node_update<P, MODE>(prm, tl, gx, gy, fx, fy, su[0][ly][lx], su[1][ly][lx]);
+      }
     } else {
       prm.f[(long)gy * prm.pitch + gx] = R2{fx, fy};
     }
