@@ -37,7 +37,7 @@ namespace se2w {
 // one per problem.
 struct RegSchedule { int ex, ey, waves, pf, lpe; };
 constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
-                                         {16, 4, 0, 0, 1}, {16, 8, 0, 0, 2}, {32, 3, 0, 0, 2}, {16, 4, 0, 0, 2}};
+                                         {16, 6, 0, 0, 2}, {16, 8, 0, 0, 2}, {16, 4, 0, 2, 2}, {16, 4, 0, 0, 2}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
 // Schedules instantiated for (p, fused): all of them for p = 4, the first
