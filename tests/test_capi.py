@@ -50,3 +50,30 @@ def test_c_headers_compile_as_c99():
     r = subprocess.run(["gcc", "-std=c99", "-Wall", "-Werror", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
                         "-x", "c", "-"], input=src, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_missing_library_raises(tmp_path):
+    """Without the built library the package raises; it has no CPU fallback."""
+    code = ("import se2dr_fork_amd as S\n"
+            f"S.LIB_DIR = {str(tmp_path)!r}\n"
+            "try:\n"
+            "    S.libraries()\n"
+            "except S.Se2wError as e:\n"
+            "    print('raised:', e)\n"
+            "else:\n"
+            "    raise SystemExit('no error')\n")
+    r = subprocess.run(["python", "-c", code], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "is missing" in r.stdout
+
+
+def test_context_without_gpu_raises(se2w):
+    """On a host without a GPU, creating a step context fails with the HIP
+    error instead of running anywhere else."""
+    try:
+        ctx = se2w.Context(mx=4, my=4, p=2, precision=se2w.FP32)
+    except se2w.Se2wError as e:
+        assert "HIP" in str(e) or "device" in str(e), e
+        return
+    ctx.close()
+    pytest.skip("a GPU is present")
