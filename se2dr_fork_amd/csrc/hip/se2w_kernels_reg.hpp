@@ -67,6 +67,14 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
   __shared__ float2 tedge[EY][EX][NB];  // top edge (j = p), i = 0..p
   // nodal forces of the owned nodes; column I stored at I + I/P
   __shared__ float2 sf[TEY * P + 1][TEX * P + 1 + TEX + 1];
+  // p >= 5: the quadrature weights in LDS (uniform reads) rather than in
+  // SGPRs, which the (p+1)^2 derivative values already nearly fill.
+  constexpr bool W_LDS = P >= 5;
+  __shared__ float sWq[W_LDS ? NB * NB : 1];
+  if constexpr (W_LDS) {
+    for (int k = threadIdx.x; k < NB * NB; k += EX * EY * LPE) sWq[k] = prm.w2[k];
+    __syncthreads();
+  }
 
   const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
   const int ex0 = tx * TEX, ey0 = ty * TEY;
@@ -181,7 +189,7 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
           dyux = fmaf(D[qj * NB + k], ux[k][qi], dyux);
           dyuy = fmaf(D[qj * NB + k], uy[k][qi], dyuy);
         }
-        const float w = W[qi + qj * NB];
+        const float w = W_LDS ? sWq[qi + qj * NB] : W[qi + qj * NB];
         A[qi] = w * fmaf(a1, dxux, a2 * dyuy);
         E[qi] = w * fmaf(a3, dyuy, a2 * dxux);
         B[qi] = w * fmaf(b1, dyux, b2 * dxuy);
@@ -199,7 +207,7 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
           dy = fmaf(D[qj * NB + k], ux[k][qi], dy);
         }
         const float odx = pair_swap(dx), ody = pair_swap(dy);
-        const float w = W[qi + qj * NB];
+        const float w = W_LDS ? sWq[qi + qj * NB] : W[qi + qj * NB];
         R[qi] = w * fmaf(rA, dx, rB * ody);
         Cq[qi] = w * fmaf(cA, dy, cB * odx);
       }
