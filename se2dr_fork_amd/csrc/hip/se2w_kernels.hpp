@@ -412,6 +412,42 @@ __device__ __forceinline__ void node_update(const StepParams<typename RealOf<MOD
   node_update_core<P, MODE>(prm, tl, gx, gy, idx, fx, fy, ux, uy, prm.v[idx], prm.mass[idx], cc);
 }
 
+// Owned node k of a tile (OX x OY owned nodes starting on an element corner)
+// in class order: first the nodes inside one element column and row, then
+// those on element-column edges (x a multiple of p), on element-row edges,
+// and the corners.  Within a class the same number of elements meets at every
+// node (except on the mesh boundary), so the threads of a wave take the same
+// path through the sum over elements.
+template <int P>
+__device__ __forceinline__ void owned_node(int k, int OX, int OY, int &ox, int &oy) {
+  const int nxe = (OX - 1) / P + 1, nye = (OY - 1) / P + 1;  // edge columns / rows
+  const int nxi = OX - nxe, nyi = OY - nye;                  // interior columns / rows
+  int c = k;
+  if constexpr (P > 1) {
+    auto inner = [](int i) { return (i / (P - 1)) * P + i % (P - 1) + 1; };
+    if (c < nxi * nyi) {
+      ox = inner(c % nxi);
+      oy = inner(c / nxi);
+      return;
+    }
+    c -= nxi * nyi;
+    if (c < nxe * nyi) {
+      ox = (c % nxe) * P;
+      oy = inner(c / nxe);
+      return;
+    }
+    c -= nxe * nyi;
+    if (c < nxi * nye) {
+      ox = inner(c % nxi);
+      oy = (c / nxi) * P;
+      return;
+    }
+    c -= nxi * nye;
+  }
+  ox = (c % nxe) * P;
+  oy = (c / nxe) * P;
+}
+
 // ---------------------------------------------------------------------------
 // Tile kernel.  FUSED: element operator + update in one pass (one launch per
 // step).  !FUSED: element operator only, f written for the update kernel.
@@ -467,14 +503,17 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
   constexpr int NSLOT = ((TEX * P + 1) * (TEY * P + 1) + NT - 1) / NT;  // owned nodes per thread (max)
   constexpr bool EARLY = P <= 6;
+  // the thread's owned nodes, in class order (see owned_node); slots past the
+  // owned block repeat its last node
+  int sox[NSLOT], soy[NSLOT];
+#pragma unroll
+  for (int sl = 0; sl < NSLOT; sl++) owned_node<P>(min(tid + sl * NT, OX * OY - 1), OX, OY, sox[sl], soy[sl]);
   R2 pv[NSLOT], pc[NSLOT];
   R pm[NSLOT];
   auto node_loads = [&]() {
 #pragma unroll
     for (int sl = 0; sl < NSLOT; sl++) {
-      const int k = min(tid + sl * NT, OX * OY - 1);
-      const int oy = k / OX, ox = k - oy * OX;
-      const long idx = (long)(ey0 * P + oy) * prm.pitch + ex0 * P + ox;
+      const long idx = (long)(ey0 * P + soy[sl]) * prm.pitch + ex0 * P + sox[sl];
       pv[sl] = prm.v[idx];
       pm[sl] = prm.mass[idx];
       if constexpr (MODE == MODE_F32K) pc[sl] = prm.c_in[idx];
@@ -524,10 +563,8 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   const TileLists tl = tile_lists(prm, tile);
 #pragma unroll
   for (int sl = 0; sl < NSLOT; sl++) {
-    const int k = tid + sl * NT;
-    if (k >= OX * OY) break;
-    const int oy = k / OX, ox = k - oy * OX;
-    const int gx = ex0 * P + ox, gy = ey0 * P + oy;
+    if (tid + sl * NT >= OX * OY) break;
+    const int gx = ex0 * P + sox[sl], gy = ey0 * P + soy[sl];
     int exs[2], eys[2], nex = 0, ney = 0;
     if (gx % P == 0) {
       if (gx / P - 1 >= 0) exs[nex++] = gx / P - 1;
