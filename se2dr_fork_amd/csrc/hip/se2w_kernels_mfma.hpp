@@ -314,7 +314,8 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
   for (int k = tid; k < OX * OY; k += kThreads) {
-    const int oy = k / OX, ox = k - oy * OX;
+    int ox, oy;
+    owned_node<P>(k, OX, OY, ox, oy);  // class order: uniform element count per wave
     const int gx = ex0 * P + ox, gy = ey0 * P + oy;
     int exs[2], eys[2], nex = 0, ney = 0;
     if (gx % P == 0) {
