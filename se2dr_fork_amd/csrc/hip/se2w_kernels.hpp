@@ -135,29 +135,29 @@ __device__ __forceinline__ TileLists tile_lists(const StepParams<R> &prm, int ti
 
 // FP64, operation order of the reference's dense loop (se2wave.c:1357-1400),
 // with the zero products of the padded tensor tables left out (they only add
-// signed zeros).  dN_dx = iJ00*D[qi][i] on row qj, dN_dy = iJ11*D[qj][j] on
-// column qi.
+// signed zeros).  dN_dx = iJ00*Di[i] on row qj, dN_dy = iJ11*Dj[j] on column
+// qi, where Di = D[qi][.] and Dj = D[qj][.].
 template <int P>
 __device__ __forceinline__ void stress_f64(const double *ux_row, const double *ux_col, const double *uy_row,
-                                           const double *uy_col, int pitch, int qi, int qj, const double *D,
-                                           double iJ00, double iJ11, double lam, double mu, double &sxx,
-                                           double &syy, double &sxy) {
+                                           const double *uy_col, int pitch, int qi, int qj, const double *Di,
+                                           const double *Dj, double iJ00, double iJ11, double lam, double mu,
+                                           double &sxx, double &syy, double &sxy) {
 #pragma clang fp contract(off)
   constexpr int NB = P + 1;
   double e0 = 0.0, e1 = 0.0, e2 = 0.0;
 #pragma unroll
-  for (int i = 0; i < NB; i++) e0 = e0 + (iJ00 * D[qi * NB + i]) * ux_row[i];
+  for (int i = 0; i < NB; i++) e0 = e0 + (iJ00 * Di[i]) * ux_row[i];
 #pragma unroll
-  for (int j = 0; j < NB; j++) e1 = e1 + (iJ11 * D[qj * NB + j]) * uy_col[j * pitch];
+  for (int j = 0; j < NB; j++) e1 = e1 + (iJ11 * Dj[j]) * uy_col[j * pitch];
 #pragma unroll
   for (int j = 0; j < NB; j++) {
     if (j != qj) {
-      e2 = e2 + (iJ11 * D[qj * NB + j]) * ux_col[j * pitch];
+      e2 = e2 + (iJ11 * Dj[j]) * ux_col[j * pitch];
     } else {
 #pragma unroll
       for (int i = 0; i < NB; i++) {
-        double t = (iJ00 * D[qi * NB + i]) * uy_row[i];
-        if (i == qi) t = t + (iJ11 * D[qj * NB + qj]) * ux_col[qj * pitch];
+        double t = (iJ00 * Di[i]) * uy_row[i];
+        if (i == qi) t = t + (iJ11 * Dj[j]) * ux_col[j * pitch];  // j == qj
         e2 = e2 + t;
       }
     }
@@ -169,23 +169,24 @@ __device__ __forceinline__ void stress_f64(const double *ux_row, const double *u
 }
 
 // FP32: sum-factorised contractions with the Jacobian applied once, stress
-// pre-multiplied by the quadrature factor detJ*w_q.
+// pre-multiplied by the quadrature factor detJ*w_q.  Di = D[qi][.], Dj =
+// D[qj][.].
 template <int P>
 __device__ __forceinline__ void stress_f32(const float *ux_row, const float *ux_col, const float *uy_row,
-                                           const float *uy_col, int pitch, int qi, int qj, const float *D,
+                                           const float *uy_col, int pitch, const float *Di, const float *Dj,
                                            float iJ00, float iJ11, float lam, float mu, float fac, float &sxx,
                                            float &syy, float &sxy) {
   constexpr int NB = P + 1;
   float dxux = 0.f, dxuy = 0.f, dyux = 0.f, dyuy = 0.f;
 #pragma unroll
   for (int i = 0; i < NB; i++) {
-    const float d = D[qi * NB + i];
+    const float d = Di[i];
     dxux = fmaf(d, ux_row[i], dxux);
     dxuy = fmaf(d, uy_row[i], dxuy);
   }
 #pragma unroll
   for (int j = 0; j < NB; j++) {
-    const float d = D[qj * NB + j];
+    const float d = Dj[j];
     dyux = fmaf(d, ux_col[j * pitch], dyux);
     dyuy = fmaf(d, uy_col[j * pitch], dyuy);
   }
@@ -533,30 +534,60 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   }
   __syncthreads();
   // Phase B: stress at the GLL points of the tile elements (+ halo).
-  for (unsigned k = tid; k < (unsigned)(QX * QY); k += NT) {
-    const unsigned qy = k / QX, qx = k - qy * QX;
-    const unsigned elx = qx / NB, qi = qx - elx * NB;
-    const unsigned ely = qy / NB, qj = qy - ely * NB;
-    const int ex = ex0 - 1 + (int)elx, ey = ey0 - 1 + (int)ely;
-    if (ex < 0 || ex >= prm.mx || ey < 0 || ey >= prm.my) continue;
-    const unsigned ei = ely * (TEX + 1) + elx;
-    const unsigned lx0 = elx * P, ly0 = ely * P;
-    const R *uxr = &su[0][ly0 + qj][lx0];
-    const R *uyr = &su[1][ly0 + qj][lx0];
-    const R *uxc = &su[0][ly0][lx0 + qi];
-    const R *uyc = &su[1][ly0][lx0 + qi];
-    R sxx, syy, sxy;
-    if constexpr (MODE == MODE_F64) {
-      stress_f64<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, econ[0][ei], econ[1][ei], econ[3][ei], econ[4][ei], sxx, syy,
+  // FP32 modes: each thread keeps one quadrature point (qi, qj) for all its
+  // elements, so the two rows of D it needs stay in registers and only the
+  // displacements are read from LDS.  FP64: the points of the tile are dealt
+  // out evenly (the heavier per-point work favours balance over the table
+  // reads).
+  if constexpr (MODE == MODE_F64) {
+    for (unsigned k = tid; k < (unsigned)(QX * QY); k += NT) {
+      const unsigned qy = k / QX, qx = k - qy * QX;
+      const unsigned elx = qx / NB, qi = qx - elx * NB;
+      const unsigned ely = qy / NB, qj = qy - ely * NB;
+      const int ex = ex0 - 1 + (int)elx, ey = ey0 - 1 + (int)ely;
+      if (ex < 0 || ex >= prm.mx || ey < 0 || ey >= prm.my) continue;
+      const unsigned ei = ely * (TEX + 1) + elx;
+      const unsigned lx0 = elx * P, ly0 = ely * P;
+      R sxx, syy, sxy;
+      stress_f64<P>(&su[0][ly0 + qj][lx0], &su[0][ly0][lx0 + qi], &su[1][ly0 + qj][lx0], &su[1][ly0][lx0 + qi], NXL,
+                    qi, qj, &sD[qi * NB], &sD[qj * NB], econ[0][ei], econ[1][ei], econ[3][ei], econ[4][ei], sxx, syy,
                     sxy);
-    } else {
-      const float fac = econ[2][ei] * sW[qi + qj * NB];
-      stress_f32<P>(uxr, uxc, uyr, uyc, NXL, qi, qj, sD, econ[0][ei], econ[1][ei], econ[3][ei], econ[4][ei], fac, sxx,
-                    syy, sxy);
+      ss[0][qy][qx] = sxx;
+      ss[1][qy][qx] = syy;
+      ss[2][qy][qx] = sxy;
     }
-    ss[0][qy][qx] = sxx;
-    ss[1][qy][qx] = syy;
-    ss[2][qy][qx] = sxy;
+  } else {
+    constexpr int NQ = NB * NB, GROUPS = NT / NQ;
+    static_assert(GROUPS >= 1, "a thread per quadrature point of an element");
+    if (tid < GROUPS * NQ) {
+      const int q = tid % NQ, g = tid / NQ;
+      const int qi = q % NB, qj = q / NB;
+      R Di[NB], Dj[NB];
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        Di[k] = sD[qi * NB + k];
+        Dj[k] = sD[qj * NB + k];
+      }
+      const R wq = sW[q];
+      for (int ei = g; ei < NE; ei += GROUPS) {
+        const int elx = ei % (TEX + 1), ely = ei / (TEX + 1);
+        const int ex = ex0 - 1 + elx, ey = ey0 - 1 + ely;
+        if (ex < 0 || ex >= prm.mx || ey < 0 || ey >= prm.my) continue;
+        const int lx0 = elx * P, ly0 = ely * P;
+        const R *uxr = &su[0][ly0 + qj][lx0];
+        const R *uyr = &su[1][ly0 + qj][lx0];
+        const R *uxc = &su[0][ly0][lx0 + qi];
+        const R *uyc = &su[1][ly0][lx0 + qi];
+        R sxx, syy, sxy;
+        const float fac = econ[2][ei] * wq;
+        stress_f32<P>(uxr, uxc, uyr, uyc, NXL, Di, Dj, econ[0][ei], econ[1][ei], econ[3][ei], econ[4][ei], fac, sxx,
+                      syy, sxy);
+        const int qy = ely * NB + qj, qx = elx * NB + qi;
+        ss[0][qy][qx] = sxx;
+        ss[1][qy][qx] = syy;
+        ss[2][qy][qx] = sxy;
+      }
+    }
   }
   __syncthreads();
   // Phase C (+D): owned nodes.
