@@ -197,8 +197,9 @@ int upload_real(se2w_ctx *c, void **dst, const double *src, size_t n) {
   return upload<float>(dst, tmp.data(), n, c->stream);
 }
 
-// Register-blocked kernel availability: FP32 modes, 2 <= p <= 6.
-bool reg_available(int p, int mode) { return mode != SE2W_FP64 && p >= 2 && p <= 6; }
+// Register-blocked kernel availability: FP32 modes, 2 <= p <= 8 (p = 7, 8:
+// the lane-pair schedules only, see reg_schedule_built).
+bool reg_available(int p, int mode) { return mode != SE2W_FP64 && p >= 2 && p <= kMaxP; }
 
 // Matrix-core kernel availability: FP32 modes, 6 <= p <= 8, fused schedule.
 static_assert(SE2W_KERNEL_MFMA == SE2W_KERNEL_REG + kNumRegSchedules, "variant numbering");
@@ -338,7 +339,7 @@ bool launch_reg_s(se2w_ctx *c, const StepParams<float> &prm) {
 template <int P, int MODE, bool FUSED>
 void launch_element_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
   const dim3 grid(8 * c->tiles_per_xcd);
-  if constexpr (MODE != MODE_F64 && P >= 2 && P <= 6) {
+  if constexpr (MODE != MODE_F64 && P >= 2) {
     if (is_reg_variant(c->variant) && launch_reg_s<P, MODE, FUSED, 0>(c, prm)) return;
   }
   if constexpr (MODE != MODE_F64 && P >= 6 && FUSED) {
@@ -528,7 +529,9 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
   c->mode = desc->precision;
   c->schedule = desc->schedule;
   c->device = desc->device;
-  c->variant = reg_available(c->p, c->mode) ? SE2W_KERNEL_REG : SE2W_KERNEL_LDS;
+  c->variant = reg_available(c->p, c->mode) && reg_variant_built(c->p, SE2W_KERNEL_REG, c->schedule)
+                   ? SE2W_KERNEL_REG
+                   : SE2W_KERNEL_LDS;
   set_tiling(c);
   int ierr = SE2W_OK;
   hipError_t he = hipSetDevice(c->device);
@@ -997,7 +1000,7 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if (variant == SE2W_KERNEL_LINE && !mfma_available(c->p, c->mode, c->schedule))
     return fail(SE2W_ERR_UNSUPPORTED, "line kernel needs FP32, 6 <= p <= 8 and the fused schedule");
   if (is_reg_variant(variant) && !reg_available(c->p, c->mode))
-    return fail(SE2W_ERR_UNSUPPORTED, "register-blocked kernel needs FP32 and 2 <= p <= 6");
+    return fail(SE2W_ERR_UNSUPPORTED, "register-blocked kernel needs FP32 and 2 <= p <= 8");
   if (is_reg_variant(variant) && !reg_variant_built(c->p, variant, c->schedule))
     return fail(SE2W_ERR_UNSUPPORTED, "kernel variant %d is not built for p = %d and this schedule", variant, c->p);
   HIPCHK(hipSetDevice(c->device));

@@ -125,7 +125,7 @@ int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc
       return 0;
     }
   }
-  if constexpr (MODE != MODE_F64 && P >= 2 && P <= 6)
+  if constexpr (MODE != MODE_F64 && P >= 2)
     return reg_dispatch<P, MODE, 0>(variant - SE2W_KERNEL_REG, args, grid, s, bpc, thr);
   return -1;
 }

@@ -41,10 +41,12 @@ constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
 // Schedules instantiated for (p, fused): all of them for p = 4, the first
-// four and the last (the lane-pair schedules) for the other degrees, the
-// first one for the split schedule's element kernel.
+// four and the last for p = 2, 3, 5, 6, only the lane-pair schedules 3 and the
+// last for p = 7, 8 (one lane per element would need over 400 registers);
+// the first one for the split schedule's element kernel (p <= 6).
 constexpr bool reg_schedule_built(int P, int S, bool fused) {
-  return fused ? (P == 4 || S < 4 || S == kNumRegSchedules - 1) : S == 0;
+  return P >= 7 ? (fused && (S == 3 || S == kNumRegSchedules - 1))
+                : fused ? (P == 4 || S < 4 || S == kNumRegSchedules - 1) : S == 0;
 }
 
 // Partner lane's value (lanes 2k and 2k+1 swap): DPP quad_perm [1, 0, 3, 2].

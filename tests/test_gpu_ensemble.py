@@ -78,7 +78,7 @@ def test_fp64_ensemble_bit_identical_to_oracle(oracle, p):
 
 
 @pytest.mark.parametrize("prec", [S.FP32, S.FP32_KAHAN])
-@pytest.mark.parametrize("p", [4, 6])
+@pytest.mark.parametrize("p", [4, 6, 8])
 def test_fp32_ensemble_equals_single_runs_for_every_variant(prec, p):
     scs = uneven(p, n=4, seed=200 + p)
     variants = [S.KERNEL_LDS] + [S.KERNEL_REG + k for k in range(8)] + [S.KERNEL_MFMA, S.KERNEL_LINE, S.KERNEL_LDS_WIDE]
