@@ -31,22 +31,21 @@ namespace se2w {
 
 // Schedules of the register-blocked kernel: workgroup shape (EX x EY
 // elements), a minimum number of waves per SIMD (0 = compiler's choice, which
-// caps the registers per lane), the node-phase loads issued before the force
-// exchange (pf 2: u, v, M and the compensation; 1: u, v, M; 0: none, all
-// loaded in the node phase) and the lanes per element.  The autotuner picks
-// one per problem.
+// caps the registers per lane), the placement of the node-phase loads (pf, see
+// reg_body) and the lanes per element.  The autotuner picks one per
+// problem.
 struct RegSchedule { int ex, ey, waves, pf, lpe; };
 constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
-                                         {16, 6, 0, 0, 2}, {16, 8, 0, 0, 2}, {16, 4, 0, 2, 2}, {16, 4, 0, 0, 2}};
+                                         {16, 4, 0, 4, 2}, {16, 8, 0, 0, 2}, {16, 4, 0, 2, 2}, {16, 4, 0, 0, 2}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
 // Schedules instantiated for (p, fused): all of them for p = 4, the first
-// four and the last for p = 2, 3, 5, 6, only the lane-pair schedules 3 and the
-// last for p = 7, 8 (one lane per element would need over 400 registers);
-// the first one for the split schedule's element kernel (p <= 6).
+// five and the last for p = 2, 3, 5, 6, only the lane-pair schedules 3, 4 and
+// the last for p = 7, 8 (one lane per element would need over 400
+// registers); the first one for the split schedule's element kernel (p <= 6).
 constexpr bool reg_schedule_built(int P, int S, bool fused) {
-  return P >= 7 ? (fused && (S == 3 || S == kNumRegSchedules - 1))
-                : fused ? (P == 4 || S < 4 || S == kNumRegSchedules - 1) : S == 0;
+  return P >= 7 ? (fused && (S == 3 || S == 4 || S == kNumRegSchedules - 1))
+                : fused ? (P == 4 || S < 5 || S == kNumRegSchedules - 1) : S == 0;
 }
 
 // Partner lane's value (lanes 2k and 2k+1 swap): DPP quad_perm [1, 0, 3, 2].
@@ -103,12 +102,13 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
   // a multiple of 64 lanes: those waves take a shorter path (top edge only).
   constexpr bool HALO_WAVES = (EX * LPE) % 64 == 0;
 
-  // The node phase's loads do not depend on the forces.  All of a thread's
-  // node loads are issued together, ahead of the updates: pf 3 before the
+  // The node phase's loads do not depend on the forces.  A thread's node
+  // loads are issued together, ahead of the updates: pf 3 before the
   // element's loads, pf 2 after the element operator (their latency overlaps
   // the force exchange), pf 0 after the exchange barrier (overlapping the
   // owners' sums); pf 1 splits u, v, M (as pf 2) from the compensation (as
-  // pf 0).
+  // pf 0); pf 4 issues them in the node phase, CHUNK slots at a time, which
+  // caps the registers they hold (high p, many node slots per thread).
   float2 pu[RPW][CPL], pv[RPW][CPL], pc[RPW][CPL];
   float pm[RPW][CPL];
   // Slots past the tile's owned block load its last owned node instead (in
@@ -303,7 +303,9 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
   }
   __syncthreads();
   const TileLists tl = tile_lists(prm, tile);
-  if constexpr (FUSED && PF < 2) {
+  constexpr int NSL = RPW * CPL, CHUNK = 8;
+  constexpr bool CHUNKED = PF == 4;
+  if constexpr (FUSED && PF < 2 && !CHUNKED) {
 #pragma unroll
     for (int r = 0; r < RPW; r++)
 #pragma unroll
@@ -352,24 +354,34 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
     }
   }
   __syncthreads();
-  // Node phase: coalesced update of the owned nodes from the prefetched values.
-#pragma unroll
-  for (int r = 0; r < RPW; r++) {
-#pragma unroll
-    for (int cidx = 0; cidx < CPL; cidx++) {
-      const int lyn = wave + r * NWAVES, lxn = lane + cidx * LANES;
-      if (lyn < OY && lxn < OX) {
-        const int gx = ex0 * P + lxn, gy = ey0 * P + lyn;
-        const long idx = (long)gy * prm.pitch + gx;
-        const float2 f = sf[lyn][lxn + lxn / P];
-        if constexpr (FUSED) {
-          node_update_core<P, MODE>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
-                                    pm[r][cidx], pc[r][cidx]);
-        } else {
-          prm.f[idx] = f;
-        }
+  // Node phase: coalesced update of the owned nodes from the loaded values.
+  auto node_step = [&](int r, int cidx) {
+    const int lyn = wave + r * NWAVES, lxn = lane + cidx * LANES;
+    if (lyn < OY && lxn < OX) {
+      const int gx = ex0 * P + lxn, gy = ey0 * P + lyn;
+      const long idx = (long)gy * prm.pitch + gx;
+      const float2 f = sf[lyn][lxn + lxn / P];
+      if constexpr (FUSED) {
+        node_update_core<P, MODE>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
+                                  pm[r][cidx], pc[r][cidx]);
+      } else {
+        prm.f[idx] = f;
       }
     }
+  };
+  if constexpr (FUSED && CHUNKED) {
+#pragma unroll
+    for (int c0 = 0; c0 < NSL; c0 += CHUNK) {
+#pragma unroll
+      for (int sl = c0; sl < c0 + CHUNK && sl < NSL; sl++) node_loads(sl / CPL, sl % CPL, true, true);
+#pragma unroll
+      for (int sl = c0; sl < c0 + CHUNK && sl < NSL; sl++) node_step(sl / CPL, sl % CPL);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int cidx = 0; cidx < CPL; cidx++) node_step(r, cidx);
   }
 }
 
