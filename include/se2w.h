@@ -129,7 +129,8 @@ int se2w_get_timing(se2w_ctx *ctx, double *run_ms, double kernel_ms[2]);
 /* Element-kernel variants: SE2W_KERNEL_LDS, or SE2W_KERNEL_REG + k for the
  * k-th schedule (workgroup shape, register cap, placement of the node loads,
  * one lane or a lane pair per element) of the register-blocked FP32 kernel
- * (2 <= p <= 6); se2w_kernel_count() variants in all. */
+ * (2 <= p <= 8; at p = 7, 8 only the lane-pair schedules);
+ * se2w_kernel_count() variants in all. */
 #define SE2W_KERNEL_LDS 0 /* LDS-staged tile kernel (all modes, 1 <= p <= 8) */
 #define SE2W_KERNEL_REG 1 /* register-blocked FP32 kernel, first schedule */
 #define SE2W_KERNEL_MFMA 9 /* FP32, 6 <= p <= 8, fused: element contractions on the matrix cores */
