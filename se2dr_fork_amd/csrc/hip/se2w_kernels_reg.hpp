@@ -3,8 +3,9 @@
 // One element per thread (LPE = 1) or per pair of adjacent lanes (LPE = 2).
 // The element's (p+1)^2 displacements are loaded from HBM/L2 into registers;
 // the tensor-product contractions run row by row with the derivative matrix
-// and quadrature weights in scalar registers (the same for every lane), and
-// the nodal forces accumulate in registers.  With LPE = 2 each lane of a pair
+// in scalar registers (the same for every lane; the quadrature weights too
+// for p <= 4, from LDS above), and the nodal forces accumulate in
+// registers.  With LPE = 2 each lane of a pair
 // holds one displacement component and accumulates one force component: the
 // lane computes the x and y derivatives of its component, swaps them with its
 // partner (one DPP move each) and forms the two stress combinations its force
