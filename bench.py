@@ -324,6 +324,28 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
     return out
 
 
+def accuracy_study(S, SC, nsteps_list, mx=512, p=4):
+    """Receiver error of the FP32 modes against FP64 on the benchmark
+    workload as the run grows: one run per precision of max(nsteps_list)
+    steps, errors over the first n steps of the traces for each n."""
+    n_max = max(nsteps_list)
+    sc = SC.config2(nt=n_max, mx=mx, p=p)
+    traces = {}
+    for name in ("fp64", "fp32", "fp32k"):
+        log(f"accuracy: {name}, {n_max} steps")
+        b = SC.build(sc, precision=S.PRECISIONS[name])
+        _, traces[name] = b.solver.advance(n_max, b.dt)
+        dt = b.dt
+        b.solver.close()
+    out = {"mx": mx, "p": p, "dt": dt, "rows": []}
+    for n in sorted(nsteps_list):
+        row = {"steps": n, "simulated_s": n * dt}
+        for name in ("fp32", "fp32k"):
+            row[name] = field_errors(traces[name][:n], traces["fp64"][:n])
+        out["rows"].append(row)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -341,6 +363,9 @@ def main():
     ap.add_argument("--variants", nargs=3, type=int, metavar=("MX", "P", "STEPS"),
                     help="only time every element-kernel variant at MX x MX elements of degree P for STEPS steps "
                          "(2048 8 STEPS: the layered 2048^2 p=8 workload) and print that as JSON")
+    ap.add_argument("--accuracy", default="",
+                    help="comma-separated step counts: receiver error of FP32 and FP32+Kahan against FP64 "
+                         "on the benchmark workload after each")
     ap.add_argument("--variants-check", type=int, default=0,
                     help="with --variants: receiver error of each variant after this many steps against FP64")
     ap.add_argument("--ensemble-steps", type=int, default=1500)
@@ -358,6 +383,9 @@ def main():
     if args.variants:
         mx, p, steps = args.variants
         print(json.dumps(kernel_variant_bench(S, SC, mx, p, steps, ("fp32", "fp32k"), args.variants_check)))
+        return
+    if args.accuracy:
+        print(json.dumps(accuracy_study(S, SC, [int(x) for x in args.accuracy.split(",")], mx=args.mx, p=args.p)))
         return
 
     sc = SC.config2(nt=max(args.full_steps, 1), mx=args.mx, p=args.p)
