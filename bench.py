@@ -10,7 +10,11 @@ are resident in HBM when the timed region starts.
 Prints one JSON line: GLL node-updates/s of the FP32 (compensated) path,
 with the FP64 path, the accuracy of FP32 against FP64 over the full
 2000-step run, the roofline of the dominant kernel and the reference
-algorithm timed on the host CPU (oracle/).
+algorithm timed on the host CPU (oracle/), plus the ensemble and surrogate
+workloads.  Other legs (one JSON line each): --variants MX P STEPS times
+every element-kernel variant at MX x MX elements of degree P; --accuracy
+N1,N2,... reports the FP32 modes' receiver error against FP64 as the run
+grows.  --gpus N > 1 exits: the build runs on one GPU.
 """
 import argparse
 import json
