@@ -378,8 +378,9 @@ def main():
                     help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is a design note only "
-                                   "(see DESIGN.md), so --gpus must be 1"}))
+        print(json.dumps({"error": "this build runs on one GPU (the step would shard by element-aligned "
+                                   "blocks with one interface-force exchange per step, not built), so --gpus "
+                                   "must be 1"}))
         sys.exit(2)
     import se2dr_fork_amd as S
     from se2dr_fork_amd import scenarios as SC
