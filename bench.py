@@ -378,7 +378,7 @@ def main():
                     help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU (the step would shard by element-aligned "
+        print(json.dumps({"error": "the benchmark runs on one GPU (the step would shard by element-aligned "
                                    "blocks with one interface-force exchange per step, not built), so --gpus "
                                    "must be 1"}))
         sys.exit(2)

@@ -8,7 +8,7 @@
  * The reference has no counterpart (it only runs the solver); the surrogate
  * maps a source position and a coarse velocity-perturbation grid to receiver
  * seismograms, and the solver (se2w.h) both generates its training data and
- * judges its error.
+ * measures its error.
  *
  * Shapes: widths[0] inputs (any size; padded to a multiple of 32 inside),
  * hidden and output widths multiples of 64, batch a multiple of 64.
