@@ -548,9 +548,12 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
         (ierr = dev_alloc(&c->A, field)))
       goto bad;
     if (c->mode == SE2W_FP32_KAHAN) {
-      if ((ierr = dev_alloc(&c->C[0], field)) || (ierr = dev_alloc(&c->C[1], field))) goto bad;
+      // One compensation buffer, updated in place (C[1] aliases C[0]): a
+      // node's compensation is read and written only by the thread that
+      // updates the node, unlike u, whose halo other tiles read.
+      if ((ierr = dev_alloc(&c->C[0], field))) goto bad;
+      c->C[1] = c->C[0];
       hipMemsetAsync(c->C[0], 0, field, c->stream);
-      hipMemsetAsync(c->C[1], 0, field, c->stream);
     }
     if (c->schedule == SE2W_SCHED_SPLIT) {
       if ((ierr = dev_alloc(&c->F, field))) goto bad;
@@ -580,6 +583,7 @@ bad:
 int se2w_destroy(se2w_ctx *c) {
   if (!c) return SE2W_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->C[1] == c->C[0]) c->C[1] = nullptr;  // aliased compensation buffer
   void **bufs[] = {&c->U[0], &c->U[1], &c->C[0], &c->C[1], &c->V, &c->A, &c->F, &c->Utmp, &c->mass, &c->lambda,
                    &c->mu, &c->edx, &c->edy, &c->D, &c->w2, &c->d_src_vals, &c->d_psi, &c->d_rec, &c->d_erec};
   for (void **b : bufs) dev_free(*b);
