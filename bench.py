@@ -14,7 +14,8 @@ algorithm timed on the host CPU (oracle/), plus the ensemble and surrogate
 workloads.  Other legs (one JSON line each): --variants MX P STEPS times
 every element-kernel variant at MX x MX elements of degree P; --accuracy
 N1,N2,... reports the FP32 modes' receiver error against FP64 as the run
-grows.  --gpus N > 1 exits: the build runs on one GPU.
+grows; --split times the element and update kernels of the split schedule
+apart (also part of the default line).  --gpus N > 1 exits: one GPU only.
 """
 import argparse
 import json
@@ -35,6 +36,11 @@ BYTES_PER_NODE = {0: 2 * 8 + 2 * 8 + 8 + 2 * 8 + 2 * 8,   # FP64: 72
                   1: 2 * 4 + 2 * 4 + 4 + 2 * 4 + 2 * 4,   # FP32: 36
                   2: 2 * 4 + 2 * 4 + 4 + 2 * 4 + 2 * 4 + 2 * 2 * 4}  # FP32 + Kahan: 52
 CANONICAL_B_NODE = {0: 152, 1: 76, 2: 92}  # two-pass scheme bytes/node/step
+# split schedule, algorithmic HBM bytes per node and step of each kernel:
+# element kernel reads u, writes f; update kernel reads f, u, v, M (+ the
+# compensation) and writes u, v (+ the compensation)
+SPLIT_ELEMENT_BYTES = {0: 2 * 8 + 2 * 8, 1: 2 * 4 + 2 * 4, 2: 2 * 4 + 2 * 4}
+SPLIT_UPDATE_BYTES = {0: 3 * 2 * 8 + 8 + 2 * 2 * 8, 1: 3 * 2 * 4 + 4 + 2 * 2 * 4, 2: 3 * 2 * 4 + 4 + 2 * 2 * 4 + 2 * 2 * 4}
 
 
 def log(msg):
@@ -165,6 +171,37 @@ def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, ker
         _, tr = bf.solver.advance(full_steps, bf.dt)
         out["trace"] = tr
         bf.solver.close()
+    return out
+
+
+def split_schedule_bench(S, SC, sc, precs, steps=100, warmup=10):
+    """Element kernel and update kernel of the split schedule timed apart
+    (HIP events around each launch on the library's stream), each against
+    the HBM peak with its own algorithmic bytes per node."""
+    nodes = sc.nnodes
+    names = {0: "fp64", 1: "fp32", 2: "fp32k"}
+    out = {}
+    for prec in precs:
+        log(f"split schedule, precision {prec}")
+        b = SC.build(sc, precision=prec, schedule=S.SCHED_SPLIT)
+        sv = b.solver
+        sv.advance(warmup, b.dt)
+        sv.ctx.set_profiling(True)
+        sv.advance(steps, b.dt)
+        run_ms, kern = sv.ctx.timing()
+        variant = sv.ctx.kernel()
+        sv.close()
+        row = {"steps": steps, "ms_per_step": run_ms / steps,
+               "element_kernel": kernel_signature(variant, sc.p, prec).replace(", true", ", false"),
+               "update_kernel": f"update_kernel<{sc.p}, {prec}>",
+               "element_kernel_ms": float(kern[0]), "update_kernel_ms": float(kern[1]),
+               "element_bytes_per_node": SPLIT_ELEMENT_BYTES[prec], "update_bytes_per_node": SPLIT_UPDATE_BYTES[prec]}
+        for k, nb in (("element", SPLIT_ELEMENT_BYTES[prec]), ("update", SPLIT_UPDATE_BYTES[prec])):
+            ms = row[k + "_kernel_ms"]
+            gbs = nb * nodes / (ms * 1e-3) / 1e9 if ms > 0 else None
+            row[k + "_GBs"] = gbs
+            row[k + "_frac"] = gbs / HBM_PEAK_GBS if gbs else None
+        out[names[prec]] = row
     return out
 
 
@@ -361,6 +398,10 @@ def main():
     ap.add_argument("--p", type=int, default=4)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-copy", action="store_true", help="skip the device copy-bandwidth measurement")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the split schedule (element and update kernels timed apart)")
+    ap.add_argument("--split", action="store_true",
+                    help="only the split schedule leg (one JSON line)")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
@@ -395,6 +436,13 @@ def main():
 
     sc = SC.config2(nt=max(args.full_steps, 1), mx=args.mx, p=args.p)
     precs = [S.PRECISIONS[x] for x in args.precisions.split(",")]
+    if args.split:
+        out = {"mx": args.mx, "p": args.p, "nodes": sc.nnodes,
+               "split_schedule": split_schedule_bench(S, SC, sc, precs, steps=args.profile_steps)}
+        if not args.no_copy:
+            out["stream_copy_GBs"] = stream_copy_gbs()
+        print(json.dumps(out))
+        return
     res = {}
     for prec in precs:
         log(f"step kernel, precision {prec}")
@@ -444,6 +492,8 @@ def main():
         "receiver_error_steps": args.full_steps,
         "algorithmic_bytes_per_launch": BYTES_PER_NODE[head] * nodes,
     }
+    if not args.no_split:
+        out["split_schedule"] = split_schedule_bench(S, SC, sc, precs)
     if not args.no_copy:
         out["stream_copy_GBs"] = stream_copy_gbs()
     if not args.no_ensemble:

@@ -40,3 +40,17 @@ def test_committed_result_line():
     c = d["cpu_baseline"]
     assert c["kind"] in ("port", "reference") and c["cores"] >= 1 and c["value"] > 0
     assert d["n_gpus"] == 1 and d["higher_is_better"] is True
+
+
+def test_split_schedule_bytes():
+    # The two kernels of the split schedule together move the two-pass
+    # scheme's bytes per node less the stored acceleration (its write and
+    # its read: 2 x 2 components), i.e. 60 B in FP32.
+    for mode, word in ((0, 8), (1, 4), (2, 4)):
+        split = bench.SPLIT_ELEMENT_BYTES[mode] + bench.SPLIT_UPDATE_BYTES[mode]
+        assert split == bench.CANONICAL_B_NODE[mode] - 2 * 2 * word
+    assert bench.SPLIT_ELEMENT_BYTES[1] + bench.SPLIT_UPDATE_BYTES[1] == 60
+    # the fused kernel drops the f round trip and the second read of u
+    for mode, word in ((0, 8), (1, 4), (2, 4)):
+        split = bench.SPLIT_ELEMENT_BYTES[mode] + bench.SPLIT_UPDATE_BYTES[mode]
+        assert split - bench.BYTES_PER_NODE[mode] == 3 * 2 * word
