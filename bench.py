@@ -106,7 +106,7 @@ def measured_traffic(mx, p, prec, signature):
 # level, lanes per element) of the register-blocked kernel's schedules, in the
 # library's variant order (SE2W_KERNEL_REG + k)
 REG_SCHEDULES = [(32, 4, 0, 2, 1), (16, 4, 0, 2, 1), (32, 2, 0, 2, 1), (32, 4, 0, 0, 2), (16, 4, 0, 4, 2),
-                 (16, 8, 0, 0, 2), (16, 4, 0, 2, 2), (16, 4, 0, 0, 2)]
+                 (16, 8, 0, 4, 2), (16, 4, 0, 2, 2), (16, 4, 0, 0, 2)]
 
 
 def kernel_signature(variant, p, mode):

@@ -37,16 +37,18 @@ namespace se2w {
 // problem.
 struct RegSchedule { int ex, ey, waves, pf, lpe; };
 constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
-                                         {16, 4, 0, 4, 2}, {16, 8, 0, 0, 2}, {16, 4, 0, 2, 2}, {16, 4, 0, 0, 2}};
+                                         {16, 4, 0, 4, 2}, {16, 8, 0, 4, 2}, {16, 4, 0, 2, 2}, {16, 4, 0, 0, 2}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
 // Schedules instantiated for (p, fused): all of them for p = 4, the first
-// five and the last for p = 2, 3, 5, 6, only the lane-pair schedules 3, 4 and
+// six and the last for p = 2, 3, 5, 6, only the lane-pair schedules 3, 4 and
 // the last for p = 7, 8 (one lane per element would need over 400
-// registers); the first one for the split schedule's element kernel (p <= 6).
+// registers; the 16 x 8 tile's LDS, 81 KB at p = 8, would leave one
+// workgroup per CU); the first one for the split schedule's element kernel
+// (p <= 6).
 constexpr bool reg_schedule_built(int P, int S, bool fused) {
   return P >= 7 ? (fused && (S == 3 || S == 4 || S == kNumRegSchedules - 1))
-                : fused ? (P == 4 || S < 5 || S == kNumRegSchedules - 1) : S == 0;
+                : fused ? (P == 4 || S < 6 || S == kNumRegSchedules - 1) : S == 0;
 }
 
 // Partner lane's value (lanes 2k and 2k+1 swap): DPP quad_perm [1, 0, 3, 2].
