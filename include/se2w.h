@@ -80,8 +80,6 @@ int se2w_set_sources(se2w_ctx *ctx, int32_t ns, const int32_t *elem, const doubl
  * node nearest to each of nr receivers. */
 int se2w_set_receivers(se2w_ctx *ctx, int32_t nr, const int32_t *elem, const int32_t *gll);
 
-/* Sets / reads the state vectors u, v, a (each 2*nx*ny doubles; NULL skips).
- * Replaces VecGetArray access to the u, v, a Vecs of se2wave_demo. */
 /* Replaces the sampling of RecordUV and RecordUV_interp (se2wave.c:2351-2581):
  * each step records u and v at every node of element elem[r] (element order
  * e = ei + ej*mx).  se2w_get_element_receivers returns, for every step k of
@@ -91,8 +89,26 @@ int se2w_set_receivers(se2w_ctx *ctx, int32_t nr, const int32_t *elem, const int
  * nearest node. */
 int se2w_set_element_receivers(se2w_ctx *ctx, int32_t n, const int32_t *elem);
 int se2w_get_element_receivers(se2w_ctx *ctx, const double *weights, double *trace);
+/* Sets / reads the state vectors u, v, a (each 2*nx*ny doubles; NULL skips).
+ * Replaces VecGetArray access to the u, v, a Vecs of se2wave_demo. */
 int se2w_set_state(se2w_ctx *ctx, const double *u, const double *v, const double *a);
 int se2w_get_state(se2w_ctx *ctx, double *u, double *v, double *a);
+/* Snapshot of u and v for the VTK writer that lets the step loop go on
+ * (replaces the VecView of u and v every -of steps, se2wave.c:4764-4770,
+ * which the reference's loop waits for).  se2w_snapshot_begin enqueues on
+ * the context's stream the conversion of the current u and v to interleaved
+ * FP64 in DMDA order into a device staging buffer, and on a second stream of
+ * the context the copy of that buffer into pinned host memory; it returns at
+ * once, and steps enqueued afterwards run while the copy proceeds.
+ * se2w_snapshot_wait (callable from another host thread, though not while
+ * se2w_snapshot_begin or se2w_destroy runs on the context) blocks until the
+ * copy is complete and returns the host arrays, 2*nx*ny doubles each, valid
+ * until the next se2w_snapshot_begin or se2w_destroy.  A begin while a
+ * snapshot is in flight first waits for it.  The staging buffers (4*nx*ny
+ * doubles on the device and as many in pinned host memory) are allocated by
+ * the first begin and kept. */
+int se2w_snapshot_begin(se2w_ctx *ctx);
+int se2w_snapshot_wait(se2w_ctx *ctx, const double **u, const double **v);
 
 /* Replaces the time loop body of se2wave_demo (se2wave.c:4722-4775) for
  * nsteps steps of size dt: predictor, SeismicSourceEvaluate with

@@ -9,7 +9,7 @@
  * All arithmetic is FP64 and follows the reference's operation order, so the
  * tables handed to the GPU are the reference's tables.
  * Status codes: 0 = success, negative = error; se2w_host_last_error() holds
- * the message (the analogue of SETERRQ's text).
+ * the calling thread's last message (the analogue of SETERRQ's text).
  */
 #ifndef SE2WAVE_H
 #define SE2WAVE_H

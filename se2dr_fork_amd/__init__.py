@@ -72,6 +72,8 @@ def _declare(hip, host):
     hip.se2w_set_receivers.argtypes = [v, i32, _ip, _ip]
     hip.se2w_set_state.argtypes = [v, _dp, _dp, _dp]
     hip.se2w_get_state.argtypes = [v, _dp, _dp, _dp]
+    hip.se2w_snapshot_begin.argtypes = [v]
+    hip.se2w_snapshot_wait.argtypes = [v, C.POINTER(_dp), C.POINTER(_dp)]
     hip.se2w_run.argtypes = [v, i32, d, _dp]
     hip.se2w_get_receivers.argtypes = [v, _dp]
     hip.se2w_assemble_forces.argtypes = [v, _dp, _dp]
@@ -549,6 +551,17 @@ class Context:
         u, v, a = (np.zeros(2 * self.nnodes) for _ in range(3))
         _gchk(self._hip.se2w_get_state(self._handle, _ptr(u), _ptr(v), _ptr(a)))
         return u, v, a
+
+    def snapshot_begin(self):
+        """Enqueue a copy of u, v (FP64, DMDA order) to pinned host memory; returns at once."""
+        _gchk(self._hip.se2w_snapshot_begin(self._handle))
+
+    def snapshot_wait(self):
+        """(u, v) of the last snapshot, copied out of the library's pinned buffer."""
+        pu, pv = _dp(), _dp()
+        _gchk(self._hip.se2w_snapshot_wait(self._handle, C.byref(pu), C.byref(pv)))
+        n = 2 * self.nnodes
+        return np.ctypeslib.as_array(pu, shape=(n,)).copy(), np.ctypeslib.as_array(pv, shape=(n,)).copy()
 
     def run(self, nsteps: int, dt: float, psi=None):
         p = None if psi is None else _ptr(np.ascontiguousarray(psi, dtype=np.float64))

@@ -160,6 +160,13 @@ struct se2w_ctx {
   bool profiling = false;
   double last_ms = 0.0, kern_ms[2] = {0.0, 0.0};
   double *d_partial = nullptr;
+  // asynchronous snapshots of u, v (FP64, DMDA order): device staging
+  // buffer, pinned host copy, the copy stream and its events
+  void *snap_dev = nullptr;
+  double *snap_host = nullptr;
+  hipStream_t copy_stream = nullptr;
+  hipEvent_t snap_ready = nullptr, snap_done = nullptr;
+  bool snap_pending = false;
 };
 
 namespace {
@@ -594,6 +601,12 @@ int se2w_destroy(se2w_ctx *c) {
     dev_free(p);
     *b = nullptr;
   }
+  if (c->copy_stream) hipStreamSynchronize(c->copy_stream);
+  dev_free(c->snap_dev);
+  if (c->snap_host) hipHostFree(c->snap_host);
+  if (c->snap_ready) hipEventDestroy(c->snap_ready);
+  if (c->snap_done) hipEventDestroy(c->snap_done);
+  if (c->copy_stream) hipStreamDestroy(c->copy_stream);
   if (c->h_stage) hipHostFree(c->h_stage);
   if (c->d_partial) hipFree(c->d_partial);
   if (c->ev0) hipEventDestroy(c->ev0);
@@ -805,6 +818,58 @@ int se2w_get_state(se2w_ctx *c, double *u, double *v, double *a) {
   }
   HIPCHK(hipStreamSynchronize(c->stream));
   hipFree(tmp);
+  return SE2W_OK;
+}
+
+int se2w_snapshot_begin(se2w_ctx *c) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->snap_pending) {  // the staging buffers are still being copied
+    HIPCHK(hipEventSynchronize(c->snap_done));
+    c->snap_pending = false;
+  }
+  const long n = (long)c->nx * c->ny;
+  const size_t field = sizeof(double) * 2 * n;
+  if (int ierr = dev_alloc(&c->snap_dev, 2 * field)) return ierr;
+  if (!c->snap_host) {
+    void *h = nullptr;
+    if (hipHostMalloc(&h, 2 * field, hipHostMallocDefault) != hipSuccess)
+      return fail(SE2W_ERR_MEM, "hipHostMalloc(%zu) failed", 2 * field);
+    c->snap_host = (double *)h;
+  }
+  if (!c->copy_stream) HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
+  if (!c->snap_ready) HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
+  if (!c->snap_done) HIPCHK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
+  double *dst[2] = {(double *)c->snap_dev, (double *)c->snap_dev + 2 * n};
+  void *src[2] = {c->U[c->cur], c->V};
+  for (int f = 0; f < 2; f++) {
+    if (c->mode == SE2W_FP64)
+      hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)src[f],
+                         (const double2 *)nullptr, dst[f], c->nx, c->ny, c->pitch);
+    else
+      hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)src[f],
+                         (const float2 *)((f == 0 && c->mode == SE2W_FP32_KAHAN) ? c->C[c->cur] : nullptr), dst[f],
+                         c->nx, c->ny, c->pitch);
+    HIPCHK(hipGetLastError());
+  }
+  HIPCHK(hipEventRecord(c->snap_ready, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
+  HIPCHK(hipMemcpyAsync(c->snap_host, c->snap_dev, 2 * field, hipMemcpyDeviceToHost, c->copy_stream));
+  HIPCHK(hipEventRecord(c->snap_done, c->copy_stream));
+  c->snap_pending = true;
+  return SE2W_OK;
+}
+
+int se2w_snapshot_wait(se2w_ctx *c, const double **u, const double **v) {
+  if (!c || !u || !v) return fail(SE2W_ERR_ARG, "null argument");
+  if (!c->snap_host) return fail(SE2W_ERR_STATE, "no snapshot taken (se2w_snapshot_begin)");
+  HIPCHK(hipSetDevice(c->device));
+  if (c->snap_pending) {
+    HIPCHK(hipEventSynchronize(c->snap_done));
+    c->snap_pending = false;
+  }
+  *u = c->snap_host;
+  *v = c->snap_host + 2 * (long)c->nx * c->ny;
   return SE2W_OK;
 }
 

@@ -37,6 +37,19 @@
     }                                                                             \
   } while (0)
 
+/* CHK for the step loop: a snapshot still being written is waited for
+   before the error returns (its thread reads the GPU context). */
+#define CHK_JOB(call)                                                             \
+  do {                                                                            \
+    int ierr_ = (call);                                                           \
+    if (ierr_) {                                                                  \
+      fprintf(stderr, "[se2wave] error %d at %s:%d: %s\n", ierr_, __FILE__, __LINE__, \
+              se2w_host_last_error());                                            \
+      se2w_vtk_job_join(&vtk_job);                                                \
+      return ierr_;                                                               \
+    }                                                                             \
+  } while (0)
+
 #define MAX_SRC 64
 #define MAX_REC 4096
 
@@ -192,6 +205,7 @@ static int run_demo(SE2WOptions opts, int mx, int my)
   int p = 2, nsources, nrecv, nt, of = 5000, found, n, s, k, prec, sched, use_internal = 1;
   int dump_ic_src = 0, ignore_receiver_output = 0, nsteps_total;
   char outdir[512] = ".";
+  SE2WVtkJob vtk_job;
 
   memset(src, 0, sizeof(src));
   memset(stf, 0, sizeof(stf));
@@ -306,18 +320,21 @@ static int run_demo(SE2WOptions opts, int mx, int my)
   trace = malloc(sizeof(double) * 100 * (nrecv > 0 ? nrecv : 1) * 7);
   if (!times || !trace) return SE2W_HOST_ERR_MEM;
   k = 0;
+  /* Snapshots (step-%.4d.vts every -of steps and at the end, 4764-4770,
+     4785-4789) are written by a second thread while the next steps run. */
+  memset(&vtk_job, 0, sizeof(vtk_job));
   while (k < nsteps_total) {
     /* advance to the next step at which the reference prints or writes */
     int next = nsteps_total, chunk;
     if ((k / 100 + 1) * 100 < next) next = (k / 100 + 1) * 100;
     if ((k / of + 1) * of < next) next = (k / of + 1) * of;
     chunk = next - k;
-    CHK(se2w_solver_advance(solver, chunk, dt));
+    CHK_JOB(se2w_solver_advance(solver, chunk, dt));
     if (!ignore_receiver_output && nrecv > 0) {
       double *tt = malloc(sizeof(double) * chunk), *tr = malloc(sizeof(double) * (size_t)chunk * nrecv * 7);
-      if (!tt || !tr) return SE2W_HOST_ERR_MEM;
-      CHK(se2w_solver_get_trace(solver, tt, tr));
-      CHK(se2w_receivers_append_rows(recv, chunk, tt, tr));
+      if (!tt || !tr) { se2w_vtk_job_join(&vtk_job); return SE2W_HOST_ERR_MEM; }
+      CHK_JOB(se2w_solver_get_trace(solver, tt, tr));
+      CHK_JOB(se2w_receivers_append_rows(recv, chunk, tt, tr));
       free(tt); free(tr);
     }
     k = next;
@@ -325,14 +342,15 @@ static int run_demo(SE2WOptions opts, int mx, int my)
     if (k % of == 0) {
       char fname[64];
       snprintf(fname, sizeof(fname), "step-%.4d.vts", k);
-      CHK(se2w_vtk_write_state(spec, se2w_solver_context(solver), outdir, fname));
+      CHK_JOB(se2w_vtk_job_start(&vtk_job, spec, se2w_solver_context(solver), outdir, fname));
     }
   }
   print_norms(se2w_solver_context(solver), k, se2w_solver_time(solver), dt);
   {
     char fname[64];
     snprintf(fname, sizeof(fname), "step-%.4d.vts", k);
-    CHK(se2w_vtk_write_state(spec, se2w_solver_context(solver), outdir, fname));
+    CHK_JOB(se2w_vtk_job_start(&vtk_job, spec, se2w_solver_context(solver), outdir, fname));
+    CHK(se2w_vtk_job_join(&vtk_job));
   }
   free(times); free(trace); free(xr);
   se2w_solver_destroy(&solver);

@@ -10,7 +10,8 @@
 
 #include "host_internal.h"
 
-static char host_err[512];
+/* per thread: the driver writes snapshots from a second thread */
+static __thread char host_err[512];
 
 int host_fail(int code, const char *fmt, ...)
 {

@@ -5,6 +5,7 @@
  * little-endian Float64 arrays; the DMDA field names "_x", "_y" are appended
  * to the vector name as the reference's DMDASetFieldName (1143-1144) does.
  */
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -83,4 +84,57 @@ int se2w_vtk_write_state(SE2WSpec s, se2w_ctx *ctx, const char *dir, const char 
   ierr = se2w_vtk_write(s, dir, fname, "disp", u, "velo", v);
   free(u); free(v);
   return ierr;
+}
+
+static void *vtk_job_run(void *arg)
+{
+  SE2WVtkJob *j = (SE2WVtkJob *)arg;
+  const double *u = NULL, *v = NULL;
+  int ierr = se2w_snapshot_wait(j->ctx, &u, &v);
+  if (ierr) {
+    j->ierr = ierr;
+    snprintf(j->msg, sizeof(j->msg), "GPU: %s", se2w_last_error());
+    return NULL;
+  }
+  ierr = se2w_vtk_write(j->spec, j->path_dir, j->fname, "disp", u, "velo", v);
+  if (ierr) {
+    j->ierr = ierr;
+    snprintf(j->msg, sizeof(j->msg), "%s", se2w_host_last_error());
+  }
+  return NULL;
+}
+
+int se2w_vtk_job_join(SE2WVtkJob *j)
+{
+  if (!j || !j->active) return 0;
+  pthread_join(*(pthread_t *)j->thread, NULL);
+  free(j->thread);
+  j->thread = NULL;
+  j->active = 0;
+  if (j->ierr) return host_fail(j->ierr, "%s", j->msg);
+  return 0;
+}
+
+int se2w_vtk_job_start(SE2WVtkJob *j, SE2WSpec spec, se2w_ctx *ctx, const char *dir, const char *fname)
+{
+  int ierr;
+  if (!j || !spec || !ctx || !fname) return host_fail(SE2W_HOST_ERR_ARG, "bad vtk job arguments");
+  if ((ierr = se2w_vtk_job_join(j))) return ierr;
+  if ((ierr = se2w_snapshot_begin(ctx))) return host_fail(ierr, "GPU: %s", se2w_last_error());
+  j->spec = spec;
+  j->ctx = ctx;
+  snprintf(j->path_dir, sizeof(j->path_dir), "%s", (dir && dir[0]) ? dir : ".");
+  snprintf(j->fname, sizeof(j->fname), "%s", fname);
+  j->ierr = 0;
+  j->msg[0] = 0;
+  j->thread = malloc(sizeof(pthread_t));
+  if (!j->thread) return host_fail(SE2W_HOST_ERR_MEM, "out of memory");
+  if (pthread_create((pthread_t *)j->thread, NULL, vtk_job_run, j)) {
+    free(j->thread);
+    j->thread = NULL;
+    vtk_job_run(j); /* no second thread: write the snapshot here */
+    return j->ierr ? host_fail(j->ierr, "%s", j->msg) : 0;
+  }
+  j->active = 1;
+  return 0;
 }

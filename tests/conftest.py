@@ -39,3 +39,19 @@ def se2w():
 
     S.libraries()
     return S
+
+
+def read_vts(path):
+    """Arrays of a VTK XML StructuredGrid file with raw appended data (UInt64
+    block headers, little-endian Float64): {name: flat array}."""
+    import re
+
+    data = open(path, "rb").read()
+    head, sep, raw = data.partition(b'<AppendedData encoding="raw">\n_')
+    assert sep, "no raw appended data"
+    out = {}
+    for m in re.finditer(rb'Name="([^"]+)" NumberOfComponents="(\d+)" format="appended" offset="(\d+)"', head):
+        off = int(m.group(3))
+        nbytes = int(np.frombuffer(raw, dtype="<u8", count=1, offset=off)[0])
+        out[m.group(1).decode()] = np.frombuffer(raw, dtype="<f8", count=nbytes // 8, offset=off + 8)
+    return out

@@ -12,7 +12,7 @@ import pytest
 
 import se2dr_fork_amd as S
 from se2dr_fork_amd import scenarios as SC
-from conftest import GOLDEN, ROOT, rel_l2
+from conftest import GOLDEN, ROOT, read_vts, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -224,7 +224,7 @@ def test_program_demo_run_matches_oracle(oracle, tmp_path):
     """The se2wave executable with the reference's demo options writes the
     same receiver file as the oracle run of the same problem."""
     exe = os.path.join(ROOT, "se2dr_fork_amd", "bin", "se2wave")
-    args = ["-mx", "16", "-bdegree", "3", "-tmax", "0.05", "-cfl_factor", "0.5", "-of", "100", "-output_dir", str(tmp_path)]
+    args = ["-mx", "16", "-bdegree", "3", "-tmax", "0.05", "-cfl_factor", "0.5", "-of", "4", "-output_dir", str(tmp_path)]
     r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stderr
     assert "[se2wave] Using time step size:" in r.stdout and "[step" in r.stdout
@@ -236,7 +236,44 @@ def test_program_demo_run_matches_oracle(oracle, tmp_path):
     pr.write_receiver_files(str(tmp_path / "orc"), to, ro)
     name = "closestqpsource-receiverCP-uva-16x16-p3-rank0.dat"
     assert (tmp_path / name).read_bytes() == (tmp_path / "orc" / name).read_bytes()
-    assert (tmp_path / f"step-{len(to):04d}.vts").exists()
+    # snapshots every 4 steps and at the end, written while the next steps
+    # run: the FP64 state of the oracle at those steps
+    steps = list(range(4, len(to), 4)) + [len(to)]
+    assert len(steps) >= 3
+    for k in steps:
+        _, _, uo, vo, _ = oracle.Problem(sc, dt=b.dt, nt=k).run()
+        a = read_vts(tmp_path / f"step-{k:04d}.vts")
+        for f, ref in (("disp", uo), ("velo", vo)):
+            np.testing.assert_array_equal(a[f + "_x"], ref[0::2])
+            np.testing.assert_array_equal(a[f + "_y"], ref[1::2])
+
+
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32_KAHAN])
+def test_snapshot_taken_while_steps_run(prec):
+    """se2w_snapshot_begin/wait return the state at the step they were begun,
+    with later steps enqueued behind them; a second begin reuses the buffers
+    and a begin while one is in flight waits for it."""
+    sc = SC.small(mx=9, my=7, p=4, nt=60)
+    b = SC.build(sc, precision=prec)
+    ctx = b.solver.ctx
+    with pytest.raises(S.Se2wError, match="no snapshot"):
+        ctx.snapshot_wait()
+    b.solver.advance(15, b.dt)
+    u, v, _ = ctx.get_state()
+    ctx.snapshot_begin()
+    b.solver.advance(10, b.dt)
+    su, sv = ctx.snapshot_wait()
+    np.testing.assert_array_equal(su, u)
+    np.testing.assert_array_equal(sv, v)
+    u2, v2, _ = ctx.get_state()
+    assert not np.array_equal(u2, u)
+    ctx.snapshot_begin()
+    ctx.snapshot_begin()
+    b.solver.advance(5, b.dt)
+    su, sv = ctx.snapshot_wait()
+    np.testing.assert_array_equal(su, u2)
+    np.testing.assert_array_equal(sv, v2)
+    b.solver.close()
 
 
 def test_config2_full_size(oracle):

@@ -219,3 +219,35 @@ def test_single_station_receivers(tmp_path):
     assert bb[6] == "# Time series header" and len(bb) == 10
     with pytest.raises(S.Se2wError, match="no containing element"):
         S.ReceiverUV(b.spec, [4000.0, 100.0])  # on the right boundary: truncation gives element mx
+
+
+def test_vtk_writer_layout(tmp_path):
+    """The structured-grid writer: mesh points in DMDA order with z = 0 and
+    each 2-component field split into <name>_x, <name>_y, read back exactly."""
+    import ctypes as C
+
+    from conftest import read_vts
+
+    _, host = S.libraries()
+    spec = S.SpecFE(3, 2, 2)
+    spec.scale_coords([2000.0, 1000.0], [-1000.0, 0.0])
+    n = spec.nnodes
+    assert n == (3 * 2 + 1) * (2 * 2 + 1)
+    rng = np.random.default_rng(4)
+    f0, f1 = rng.standard_normal(2 * n), rng.standard_normal(2 * n)
+    dp = C.POINTER(C.c_double)
+    host.se2w_vtk_write.argtypes = [C.c_void_p, C.c_char_p, C.c_char_p, C.c_char_p, dp, C.c_char_p, dp]
+    rc = host.se2w_vtk_write(spec.h, str(tmp_path).encode(), b"t.vts", b"disp", f0.ctypes.data_as(dp), b"velo",
+                             f1.ctypes.data_as(dp))
+    assert rc == 0
+    a = read_vts(tmp_path / "t.vts")
+    assert sorted(a) == ["Position", "disp_x", "disp_y", "velo_x", "velo_y"]
+    xy = spec.coords().reshape(-1, 2)
+    pos = a["Position"].reshape(-1, 3)
+    np.testing.assert_array_equal(pos[:, :2], xy)
+    assert not pos[:, 2].any()
+    np.testing.assert_array_equal(a["disp_x"], f0[0::2])
+    np.testing.assert_array_equal(a["disp_y"], f0[1::2])
+    np.testing.assert_array_equal(a["velo_x"], f1[0::2])
+    np.testing.assert_array_equal(a["velo_y"], f1[1::2])
+    spec.close()
