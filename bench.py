@@ -390,7 +390,7 @@ def accuracy_study(S, SC, nsteps_list, mx=512, p=4):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
+    ap.add_argument("--steps", type=int, default=2000, help="timed steps (the workload's run length)")
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--full-steps", type=int, default=2000, help="length of the accuracy runs (0: skip)")
     ap.add_argument("--profile-steps", type=int, default=100)
