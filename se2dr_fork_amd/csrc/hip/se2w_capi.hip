@@ -795,30 +795,40 @@ int se2w_set_state(se2w_ctx *c, const double *u, const double *v, const double *
   return SE2W_OK;
 }
 
+// Enqueues on the context's stream the conversion of state field f (0 = u,
+// with its compensation in FP32+Kahan; 1 = v; 2 = a) to interleaved FP64 in
+// DMDA order at the device address dst.
+static int enqueue_field_fp64(se2w_ctx *c, int f, double *dst) {
+  const long n = (long)c->nx * c->ny;
+  void *src = f == 0 ? c->U[c->cur] : f == 1 ? c->V : c->A;
+  if (c->mode == SE2W_FP64)
+    hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)src,
+                       (const double2 *)nullptr, dst, c->nx, c->ny, c->pitch);
+  else
+    hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)src,
+                       (const float2 *)((f == 0 && c->mode == SE2W_FP32_KAHAN) ? c->C[c->cur] : nullptr), dst,
+                       c->nx, c->ny, c->pitch);
+  HIPCHK(hipGetLastError());
+  return SE2W_OK;
+}
+
 int se2w_get_state(se2w_ctx *c, double *u, double *v, double *a) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
-  const long n = (long)c->nx * c->ny;
+  const size_t field = sizeof(double) * 2 * (long)c->nx * c->ny;
   double *tmp = nullptr;
-  int ierr = dev_alloc((void **)&tmp, sizeof(double) * 2 * n);
+  int ierr = dev_alloc((void **)&tmp, field);
   if (ierr) return ierr;
   double *dst[3] = {u, v, a};
-  void *src[3] = {c->U[c->cur], c->V, c->A};
-  for (int f = 0; f < 3; f++) {
+  for (int f = 0; f < 3 && !ierr; f++) {
     if (!dst[f]) continue;
-    if (c->mode == SE2W_FP64)
-      hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)src[f],
-                         (const double2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
-    else
-      hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)src[f],
-                         (const float2 *)((f == 0 && c->mode == SE2W_FP32_KAHAN) ? c->C[c->cur] : nullptr), tmp,
-                         c->nx, c->ny, c->pitch);
-    HIPCHK(hipGetLastError());
-    HIPCHK(hipMemcpyAsync(dst[f], tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+    ierr = enqueue_field_fp64(c, f, tmp);
+    if (!ierr && hipMemcpyAsync(dst[f], tmp, field, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
+      ierr = fail(SE2W_ERR_HIP, "hipMemcpyAsync of the state failed");
   }
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (hipStreamSynchronize(c->stream) != hipSuccess && !ierr) ierr = fail(SE2W_ERR_HIP, "hipStreamSynchronize failed");
   hipFree(tmp);
-  return SE2W_OK;
+  return ierr;
 }
 
 int se2w_snapshot_begin(se2w_ctx *c) {
@@ -840,18 +850,8 @@ int se2w_snapshot_begin(se2w_ctx *c) {
   if (!c->copy_stream) HIPCHK(hipStreamCreateWithFlags(&c->copy_stream, hipStreamNonBlocking));
   if (!c->snap_ready) HIPCHK(hipEventCreateWithFlags(&c->snap_ready, hipEventDisableTiming));
   if (!c->snap_done) HIPCHK(hipEventCreateWithFlags(&c->snap_done, hipEventDisableTiming));
-  double *dst[2] = {(double *)c->snap_dev, (double *)c->snap_dev + 2 * n};
-  void *src[2] = {c->U[c->cur], c->V};
-  for (int f = 0; f < 2; f++) {
-    if (c->mode == SE2W_FP64)
-      hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)src[f],
-                         (const double2 *)nullptr, dst[f], c->nx, c->ny, c->pitch);
-    else
-      hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)src[f],
-                         (const float2 *)((f == 0 && c->mode == SE2W_FP32_KAHAN) ? c->C[c->cur] : nullptr), dst[f],
-                         c->nx, c->ny, c->pitch);
-    HIPCHK(hipGetLastError());
-  }
+  for (int f = 0; f < 2; f++)
+    if (int ierr = enqueue_field_fp64(c, f, (double *)c->snap_dev + f * 2 * n)) return ierr;
   HIPCHK(hipEventRecord(c->snap_ready, c->stream));
   HIPCHK(hipStreamWaitEvent(c->copy_stream, c->snap_ready, 0));
   HIPCHK(hipMemcpyAsync(c->snap_host, c->snap_dev, 2 * field, hipMemcpyDeviceToHost, c->copy_stream));
