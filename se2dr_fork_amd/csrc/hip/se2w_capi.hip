@@ -186,6 +186,14 @@ void dev_free(void *&p) {
   p = nullptr;
 }
 
+// Device scratch of one call, freed on every return path.
+struct ScopedDev {
+  void *p = nullptr;
+  ~ScopedDev() {
+    if (p) hipFree(p);
+  }
+};
+
 template <typename T>
 int upload(void **dst, const T *src, size_t n, hipStream_t s) {
   dev_free(*dst);
@@ -659,8 +667,9 @@ int se2w_set_mass(se2w_ctx *c, const double *mass) {
   const size_t rs = real_size(c->mode);
   int ierr = dev_alloc(&c->mass, (size_t)c->pitch * c->ny * rs);
   if (ierr) return ierr;
-  double *tmp = nullptr;
-  if ((ierr = dev_alloc((void **)&tmp, sizeof(double) * n))) return ierr;
+  ScopedDev scratch;
+  if ((ierr = dev_alloc(&scratch.p, sizeof(double) * n))) return ierr;
+  double *tmp = (double *)scratch.p;
   HIPCHK(hipMemcpyAsync(tmp, mass, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
   if (c->mode == SE2W_FP64)
     hipLaunchKernelGGL(k_scalar_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp,
@@ -670,7 +679,6 @@ int se2w_set_mass(se2w_ctx *c, const double *mass) {
                        c->nx, c->ny, c->pitch, 1);
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(tmp);
   c->have_mass = true;
   return SE2W_OK;
 }
@@ -772,9 +780,9 @@ int se2w_set_state(se2w_ctx *c, const double *u, const double *v, const double *
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
   const long n = (long)c->nx * c->ny;
-  double *tmp = nullptr;
-  int ierr = dev_alloc((void **)&tmp, sizeof(double) * 2 * n);
-  if (ierr) return ierr;
+  ScopedDev scratch;
+  if (int ierr = dev_alloc(&scratch.p, sizeof(double) * 2 * n)) return ierr;
+  double *tmp = (double *)scratch.p;
   const double *src[3] = {u, v, a};
   void *dst[3] = {c->U[c->cur], c->V, c->A};
   for (int f = 0; f < 3; f++) {
@@ -791,7 +799,6 @@ int se2w_set_state(se2w_ctx *c, const double *u, const double *v, const double *
   if (u && c->mode == SE2W_FP32_KAHAN)
     HIPCHK(hipMemsetAsync(c->C[c->cur], 0, (size_t)c->pitch * c->ny * 2 * sizeof(float), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(tmp);
   return SE2W_OK;
 }
 
@@ -816,19 +823,17 @@ int se2w_get_state(se2w_ctx *c, double *u, double *v, double *a) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   HIPCHK(hipSetDevice(c->device));
   const size_t field = sizeof(double) * 2 * (long)c->nx * c->ny;
-  double *tmp = nullptr;
-  int ierr = dev_alloc((void **)&tmp, field);
-  if (ierr) return ierr;
+  ScopedDev scratch;
+  if (int ierr = dev_alloc(&scratch.p, field)) return ierr;
+  double *tmp = (double *)scratch.p;
   double *dst[3] = {u, v, a};
-  for (int f = 0; f < 3 && !ierr; f++) {
+  for (int f = 0; f < 3; f++) {
     if (!dst[f]) continue;
-    ierr = enqueue_field_fp64(c, f, tmp);
-    if (!ierr && hipMemcpyAsync(dst[f], tmp, field, hipMemcpyDeviceToHost, c->stream) != hipSuccess)
-      ierr = fail(SE2W_ERR_HIP, "hipMemcpyAsync of the state failed");
+    if (int ierr = enqueue_field_fp64(c, f, tmp)) return ierr;
+    HIPCHK(hipMemcpyAsync(dst[f], tmp, field, hipMemcpyDeviceToHost, c->stream));
   }
-  if (hipStreamSynchronize(c->stream) != hipSuccess && !ierr) ierr = fail(SE2W_ERR_HIP, "hipStreamSynchronize failed");
-  hipFree(tmp);
-  return ierr;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SE2W_OK;
 }
 
 int se2w_snapshot_begin(se2w_ctx *c) {
@@ -989,8 +994,9 @@ int se2w_assemble_forces(se2w_ctx *c, const double *u, double *f) {
   if ((ierr = dev_alloc(&c->Utmp, field))) return ierr;
   if ((ierr = dev_alloc(&c->F, field))) return ierr;
   const long n = (long)c->nx * c->ny;
-  double *tmp = nullptr;
-  if ((ierr = dev_alloc((void **)&tmp, sizeof(double) * 2 * n))) return ierr;
+  ScopedDev scratch;
+  if ((ierr = dev_alloc(&scratch.p, sizeof(double) * 2 * n))) return ierr;
+  double *tmp = (double *)scratch.p;
   HIPCHK(hipMemcpyAsync(tmp, u, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
   if (c->mode == SE2W_FP64) {
     hipLaunchKernelGGL(k_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (double2 *)c->Utmp,
@@ -1012,7 +1018,6 @@ int se2w_assemble_forces(se2w_ctx *c, const double *u, double *f) {
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(f, tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  hipFree(tmp);
   return SE2W_OK;
 }
 
