@@ -7,11 +7,16 @@ two Ricker sources (15 Hz), 16 receivers, dt = 0.2 CFL.  A step is one
 explicit Verlet step of the whole mesh with receiver sampling; the inputs
 are resident in HBM when the timed region starts.
 
-Prints one JSON line: GLL node-updates/s of the FP32 (compensated) path,
-with the FP64 path, the accuracy of FP32 against FP64 over the full
-2000-step run, the roofline of the dominant kernel and the reference
-algorithm timed on the host CPU (oracle/), plus the ensemble and surrogate
-workloads.  Other legs (one JSON line each): --variants MX P STEPS times
+Prints one JSON line: GLL node-updates/s of the FP64 path (the reference's
+arithmetic; the headline `value`), with the FP32+Kahan and FP32 paths beside
+it, the accuracy of FP32 against FP64 over the full 2000-step run, the
+roofline of the dominant kernel (here and at twice the mesh width, whose
+state exceeds the 256 MB Infinity Cache) and the reference algorithm timed
+on the host CPU (oracle/: one core and all cores, with and without the
+per-step receiver file), plus the ensemble and surrogate workloads.  The
+timed advances are three repetitions of --steps steps after --warmup steps,
+reported as their median; their buffers are sized beforehand, so none of
+them allocates.  Other legs (one JSON line each): --variants MX P STEPS times
 every element-kernel variant at MX x MX elements of degree P; --accuracy
 N1,N2,... reports the FP32 modes' receiver error against FP64 as the run
 grows; --split times the element and update kernels of the split schedule
@@ -145,17 +150,28 @@ def kernel_name(variant, p, mode):
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
 
 
-def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, kernel=None):
+def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, kernel=None, reps=3):
+    """`reps` timed advances of `steps` steps each after `warmup` untimed
+    steps; the run buffers are sized for the longest advance beforehand, and
+    the timed advances are checked to allocate nothing.  Returns the median
+    wall time, the per-launch kernel time and (full_steps > 0) the receiver
+    rows of a separate full-length run."""
     b = SC.build(sc, precision=prec)
     sv = b.solver
     if kernel is not None:
         sv.ctx.set_kernel(kernel)
+    sv.reserve(max(steps, warmup, profile_steps, 1))
+    caps = sv.ctx.reserved()
     if warmup > 0:
         sv.advance(warmup, b.dt)
-    t0 = time.perf_counter()
-    sv.advance(steps, b.dt)
-    wall = time.perf_counter() - t0
-    run_ms, _ = sv.ctx.timing()
+    walls, event_ms = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        sv.advance(steps, b.dt)
+        walls.append(time.perf_counter() - t0)
+        event_ms.append(sv.ctx.timing()[0])
+    if sv.ctx.reserved() != caps:
+        raise RuntimeError("a timed advance grew the run buffers")
     # per-launch kernel time (HIP events on the library's stream)
     sv.ctx.set_profiling(True)
     sv.advance(profile_steps, b.dt)
@@ -163,7 +179,8 @@ def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, ker
     sv.ctx.set_profiling(False)
     variant = sv.ctx.kernel()
     sv.close()
-    out = {"wall_s": wall, "event_ms": run_ms, "kernel_ms": float(kern[0]), "dt": b.dt, "variant": variant}
+    out = {"wall_s": float(np.median(walls)), "wall_s_reps": walls, "event_ms": float(np.median(event_ms)),
+           "kernel_ms": float(kern[0]), "dt": b.dt, "variant": variant}
     if full_steps > 0:
         bf = SC.build(sc, precision=prec)
         if kernel is not None:
@@ -205,32 +222,74 @@ def split_schedule_bench(S, SC, sc, precs, steps=100, warmup=10):
     return out
 
 
-def cpu_baseline(sc, dt, seconds_target=15.0):
-    """The reference algorithm (dense FP64 element loop, oracle/) on the host.
+def cpu_info():
+    """(CPU model, logical CPUs of the host, CPUs this process may run on)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return model, os.cpu_count() or 1, usable
 
-    Setup (mesh, tables, mass) is excluded: a 1-step run and an n-step run
-    are timed and the rate is taken over their difference of n - 1 steps."""
+
+def cpu_rate(sc, dt, threads, seconds_target, io_dir=None):
+    """node-updates/s of the reference algorithm (dense FP64 element loop,
+    oracle/ref_cpu.c) on `threads` OpenMP threads.  Setup (mesh, tables,
+    mass) is excluded: a 1-step run and an n-step run are timed and the rate
+    is taken over their difference of n - 1 steps.  io_dir: every step appends
+    its receiver row to the receiver file there (open, write, close), as the
+    reference does; otherwise no receiver output (-ignore_receiver_output)."""
     from oracle import oracle as O
-    cores = min(os.cpu_count() or 1, 16)
 
     def timed(n):
-        pr = O.Problem(sc, dt=dt, nthreads=cores, nt=n)
+        pr = O.Problem(sc, dt=dt, nthreads=threads, nt=n, rec_dir=io_dir)
         t0 = time.perf_counter()
-        pr.run(max_steps=n, want_state=False)
+        pr.run(max_steps=n, want_state=False, want_trace=False)
         return time.perf_counter() - t0
 
     t1 = timed(1)
     t3 = timed(3)
     per_step = max((t3 - t1) / 2, 1e-4)
-    n = int(max(4, min(1000, seconds_target / per_step)))
+    n = int(max(4, min(1000, 1 + seconds_target / per_step)))
     tn = timed(n)
     el = max(tn - t1, 1e-9)
-    return {"value": sc.nnodes * (n - 1) / el, "unit": "node-updates/s", "cores": cores, "kind": "port",
-            "sample": f"{n - 1} steps of the {sc.mx}x{sc.my} p={sc.p} workload (FP64 dense reference element loop, "
-                      f"oracle/ref_cpu.c, {cores} OpenMP threads): {n}-step run minus 1-step run = {el:.1f} s"}
+    return sc.nnodes * (n - 1) / el, n - 1, el
 
 
-def ensemble_bench(S, SC, n=256, nt=1500):
+def cpu_baseline(sc, dt, seconds_target=8.0):
+    """The reference algorithm on the host's cores: one core, and all the
+    cores this process may use (at most 16, the box's share), compute only
+    and with the reference's per-step receiver file append."""
+    import tempfile
+
+    model, nproc, usable = cpu_info()
+    env_threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    cores = max(1, min(usable, env_threads or usable, 16))
+    one, n1, el1 = cpu_rate(sc, dt, 1, seconds_target)
+    allc, na, ela = cpu_rate(sc, dt, cores, seconds_target)
+    with tempfile.TemporaryDirectory() as d:
+        allio, nio, elio = cpu_rate(sc, dt, cores, seconds_target, io_dir=d)
+    return {"value": allc, "unit": "node-updates/s", "cores": cores, "kind": "port",
+            "sample": f"{na} steps of the {sc.mx}x{sc.my} p={sc.p} workload (FP64 dense reference element loop, "
+                      f"oracle/ref_cpu.c, {cores} OpenMP threads, no receiver output): n-step run minus 1-step run "
+                      f"= {ela:.1f} s",
+            "single_core": {"value": one, "steps": n1, "seconds": el1},
+            "all_cores": {"value": allc, "threads": cores, "steps": na, "seconds": ela},
+            "all_cores_with_receiver_io": {"value": allio, "threads": cores, "steps": nio, "seconds": elio,
+                                           "what": "each step appends its 16-station row to the receiver file "
+                                                   "(fopen/fprintf/fclose), as se2wave.c:2904-2981 does"},
+            "cpu_model": model, "nproc": nproc, "usable_cpus": usable}
+
+
+def ensemble_bench(S, SC, n=256, nt=1500, n_fp64=16):
     """The uneven ensemble workload: n instances of mx, my ~ U{64..192}, p = 6,
     FP32, per-cell random materials, one Ricker source, 8 receivers, nt steps
     each; all advanced together by one batched launch per step (HIP graph),
@@ -240,6 +299,8 @@ def ensemble_bench(S, SC, n=256, nt=1500):
     builts = [SC.build(sc, precision=S.FP32) for sc in scs]
     ens = S.Ensemble([b.solver for b in builts])
     best, _ = ens.autotune(steps=10, reps=2)
+    for b in builts:
+        b.solver.reserve(nt)
     log(f"ensemble: {n} members, batched run")
     dts = [b.dt for b in builts]
     t0 = time.perf_counter()
@@ -254,11 +315,22 @@ def ensemble_bench(S, SC, n=256, nt=1500):
     log("ensemble: sequential solves")
     for i, sc in enumerate(scs):
         b = SC.build(sc, precision=S.FP32)
+        b.solver.reserve(nt)
         t0 = time.perf_counter()
         _, tr = b.solver.advance(nt, b.dt)
         t_seq += time.perf_counter() - t0
         errs.append(rel_l2(tr[:, :, :4], res[i][1][:, :, :4]))
         b.solver.close()
+    # receiver error of the batched FP32 members against FP64 solves of the
+    # same instances (a sample of the ensemble)
+    log(f"ensemble: FP64 solves of {min(n_fp64, n)} members")
+    e64 = {"u": [], "v": []}
+    for i in range(min(n_fp64, n)):
+        b = SC.build(scs[i], precision=S.FP64)
+        _, tr = b.solver.advance(nt, b.dt)
+        b.solver.close()
+        e64["u"].append(rel_l2(res[i][1][:, :, 0:2], tr[:, :, 0:2]))
+        e64["v"].append(rel_l2(res[i][1][:, :, 2:4], tr[:, :, 2:4]))
     return {"workload": f"{n} instances, mx, my ~ U{{64..192}}, p=6, FP32, per-cell random material, "
                         f"1 Ricker source (f0 ~ U[5,20] Hz), 8 receivers, {nt} steps each",
             "instances": n, "nodes": nodes, "steps": nt,
@@ -267,7 +339,40 @@ def ensemble_bench(S, SC, n=256, nt=1500):
             "node_updates_per_s": nodes * nt / t_ens,
             "kernel": ens_kernel_signature(best, 6, 1),
             "tiles": tiles, "grid": grid,
-            "max_rel_l2_u_v_vs_sequential": max(errs)}
+            "max_rel_l2_u_v_vs_sequential": max(errs),
+            "receiver_rel_l2_vs_fp64": {"members": len(e64["u"]),
+                                        "u_max": max(e64["u"]), "u_median": float(np.median(e64["u"])),
+                                        "v_max": max(e64["v"]), "v_median": float(np.median(e64["v"]))}}
+
+
+def beyond_cache_bench(S, SC, mx=1024, p=4, precs=(0, 2, 1), steps=100, warmup=10):
+    """The fused step kernel at mx x mx elements of degree p, where the state
+    (FP64: 72 B per node, 1.2 GB at 1024^2 p4) is several times the 256 MB
+    Infinity Cache, so that the kernel's bytes come from HBM: mean kernel
+    time (HIP events around `steps` back-to-back launches) against the HBM
+    peak with the kernel's algorithmic bytes per node."""
+    sc = SC.config2(nt=steps + warmup, mx=mx, p=p)
+    names = {0: "fp64", 1: "fp32", 2: "fp32k"}
+    out = {"mx": mx, "p": p, "nodes": sc.nnodes}
+    for prec in precs:
+        log(f"beyond-cache leg, {mx}x{mx} p={p}, precision {prec}")
+        b = SC.build(sc, precision=prec)
+        sv = b.solver
+        sv.reserve(steps)
+        sv.advance(warmup, b.dt)
+        sv.ctx.set_profiling(True)
+        sv.advance(steps, b.dt)
+        _, kern = sv.ctx.timing()
+        variant = sv.ctx.kernel()
+        sv.close()
+        ms = float(kern[0])
+        gbs = BYTES_PER_NODE[prec] * sc.nnodes / (ms * 1e-3) / 1e9
+        out[names[prec]] = {"kernel": kernel_signature(variant, p, prec), "kernel_ms": ms,
+                            "bytes_per_node": BYTES_PER_NODE[prec],
+                            "algorithmic_bytes_per_launch": BYTES_PER_NODE[prec] * sc.nnodes,
+                            "achieved_GBs": gbs, "frac": gbs / HBM_PEAK_GBS,
+                            "node_updates_per_s": sc.nnodes / (ms * 1e-3)}
+    return out
 
 
 def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
@@ -414,7 +519,10 @@ def main():
     ap.add_argument("--variants-check", type=int, default=0,
                     help="with --variants: receiver error of each variant after this many steps against FP64")
     ap.add_argument("--ensemble-steps", type=int, default=1500)
-    ap.add_argument("--precisions", default="fp32k,fp32,fp64")
+    ap.add_argument("--precisions", default="fp64,fp32k,fp32",
+                    help="the first one is the headline (value, dtype, roofline)")
+    ap.add_argument("--no-beyond-cache", action="store_true",
+                    help="skip the leg at twice the mesh width, whose state exceeds the Infinity Cache")
     ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
                     help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
     args = ap.parse_args()
@@ -486,12 +594,28 @@ def main():
                      "kernel": kernel_name(res[head]["variant"], args.p, head),
                      "bytes_per_node": BYTES_PER_NODE[head], "kernel_ms": kern_ms},
         "node_updates_per_s": {names[p]: r for p, r in rates.items()},
-        "canonical_roofline_frac": {names[p]: rates[p] * CANONICAL_B_NODE[p] / (HBM_PEAK_GBS * 1e9) for p in rates},
+        "ms_per_step_by_precision": {names[p]: 1e3 * r["wall_s"] / args.steps for p, r in res.items()},
+        "wall_s_reps": {names[p]: r["wall_s_reps"] for p, r in res.items()},
         "kernel_ms": {names[p]: r["kernel_ms"] for p, r in res.items()},
+        "kernels": {names[p]: kernel_signature(r["variant"], args.p, p) for p, r in res.items()},
+        "roofline_by_precision": {
+            names[p]: {"bytes_per_node": BYTES_PER_NODE[p],
+                       "achieved_GBs": BYTES_PER_NODE[p] * nodes / (r["kernel_ms"] * 1e-3) / 1e9,
+                       "frac": BYTES_PER_NODE[p] * nodes / (r["kernel_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS}
+            for p, r in res.items() if r["kernel_ms"] > 0},
+        # the two-pass scheme's bytes (76 / 92 / 152 B per node) at the wall
+        # rate: only for the 70 % of 1.05e11 node-updates/s target check, not
+        # a roofline fraction of the fused kernel (which moves fewer bytes)
+        "target_check": {names[p]: {"canonical_bytes_per_node": CANONICAL_B_NODE[p],
+                                    "canonical_frac_at_wall_rate": rates[p] * CANONICAL_B_NODE[p] / (HBM_PEAK_GBS * 1e9),
+                                    "fp32_target_node_updates_per_s": 0.7 * HBM_PEAK_GBS * 1e9 / CANONICAL_B_NODE[1]}
+                         for p in rates},
         "receiver_rel_l2_vs_fp64": {names[p]: e for p, e in errs.items()},
         "receiver_error_steps": args.full_steps,
         "algorithmic_bytes_per_launch": BYTES_PER_NODE[head] * nodes,
     }
+    if not args.no_beyond_cache:
+        out["beyond_cache"] = beyond_cache_bench(S, SC, mx=2 * args.mx, p=args.p)
     if not args.no_split:
         out["split_schedule"] = split_schedule_bench(S, SC, sc, precs)
     if not args.no_copy:

@@ -118,6 +118,21 @@ int se2w_snapshot_wait(se2w_ctx *ctx, const double **u, const double **v);
  * accumulated times. */
 int se2w_run(se2w_ctx *ctx, int32_t nsteps, double dt, const double *psi);
 
+/* Sizes, ahead of a run, the buffers a run of up to nsteps steps needs (the
+ * device source-time table, the device receiver records and the pinned host
+ * buffer of the record download), so that se2w_run and se2w_get_receivers
+ * of such a run allocate nothing.  No counterpart in the reference, whose
+ * loop allocates nothing per step: the buffers here hold a whole run.
+ * se2w_get_reserved reports the current capacities {psi values, receiver
+ * record values, element-receiver record values, pinned bytes}. */
+int se2w_reserve(se2w_ctx *ctx, int32_t nsteps);
+int se2w_get_reserved(se2w_ctx *ctx, int64_t caps[4]);
+
+/* counts = {sources, receivers, element receivers, steps of the last run}:
+ * the sizes of the psi table a run reads (nsteps * sources values) and of
+ * the rows se2w_get_receivers / se2w_get_element_receivers write. */
+int se2w_get_counts(se2w_ctx *ctx, int32_t counts[4]);
+
 /* Receiver rows of the last se2w_run: trace[(k*nr + r)*7 + c] with columns
  * ux uy vx vy ax ay curl(v), as RecordUVA_MultipleStations_NearestGLL_MPI
  * writes them (se2wave.c:2908-2959). */

@@ -200,6 +200,9 @@ se2w_ctx *se2w_solver_context(SE2WSolver solver);
  * accumulated as time = time + dt, se2wave.c:4724); receiver rows of these
  * steps are available through se2w_solver_get_trace. */
 int se2w_solver_advance(SE2WSolver solver, int nsteps, double dt);
+/* Sizes the host and device buffers of advances of up to nsteps steps
+ * (se2w_reserve), so that such an advance allocates nothing. */
+int se2w_solver_reserve(SE2WSolver solver, int nsteps);
 double se2w_solver_time(SE2WSolver solver);
 int se2w_solver_steps_done(SE2WSolver solver);
 /* times[nsteps] and trace[nsteps][nr][7] of the last advance. */

@@ -29,6 +29,9 @@ class OrcScenario(C.Structure):
         ("use_nearest_internal_gll", C.c_int),
         ("nrec", C.c_int), ("rec_xy", C.POINTER(C.c_double)),
         ("dt", C.c_double), ("nt", C.c_int), ("tmax", C.c_double), ("nthreads", C.c_int),
+        ("src_impl", C.POINTER(C.c_int)),
+        ("nuv", C.c_int), ("uv_xy", C.POINTER(C.c_double)), ("uv_interp", C.POINTER(C.c_int)),
+        ("rec_dir", C.c_char_p),
     ]
 
 
@@ -61,7 +64,10 @@ def _chk(rc):
 class Problem:
     """Holds the ctypes scenario struct and keeps its arrays alive."""
 
-    def __init__(self, sc, dt=None, nthreads=None, nt=None):
+    def __init__(self, sc, dt=None, nthreads=None, nt=None, uv_xy=None, uv_interp=None, rec_dir=None):
+        """uv_xy / uv_interp: RecordUV (0) or RecordUV_interp (1) stations;
+        rec_dir: append each step's receiver row to the receiver file in that
+        directory during run() (the reference's per-step file I/O)."""
         mu = np.asarray(sc.vs) * np.asarray(sc.vs) * np.asarray(sc.rho)  # se2wave.c:1166-1167
         lam = np.asarray(sc.vp) * np.asarray(sc.vp) * np.asarray(sc.rho) - 2.0 * mu
         self._keep = []
@@ -93,6 +99,15 @@ class Problem:
         s.nt = int(nt if nt is not None else sc.nt)
         s.tmax = float(sc.tmax)
         s.nthreads = int(nthreads or os.cpu_count() or 1)
+        impl = getattr(sc, "src_impl", None)
+        s.src_impl = _ip(keep(np.asarray(impl), np.int32)) if (impl is not None and ns) else None
+        nuv = 0 if uv_xy is None else len(uv_xy)
+        s.nuv = nuv
+        s.uv_xy = _dp(keep(np.asarray(uv_xy).reshape(-1))) if nuv else None
+        s.uv_interp = _ip(keep(np.asarray(uv_interp if uv_interp is not None else [0] * nuv), np.int32)) if nuv else None
+        self._rec_dir = None if rec_dir is None else str(rec_dir).encode()
+        s.rec_dir = self._rec_dir
+        self.nuv = nuv
         self.s = s
         self.sc = sc
         self.nnodes = sc.nnodes
@@ -132,21 +147,46 @@ class Problem:
         _chk(lib().orc_receiver_nodes(C.byref(self.s), _ip(nid), _ip(eid), _ip(g)))
         return nid[:nr], eid[:nr], g[:nr]
 
-    def run(self, max_steps=0, u0=None, v0=None, a0=None, want_state=True):
-        """Returns (times, trace[k][r][7], u, v, a)."""
+    def run(self, max_steps=0, u0=None, v0=None, a0=None, want_state=True, want_uv=False, want_trace=True):
+        """Returns (times, trace[k][r][7], u, v, a), and uv_trace[k][r][4] of
+        the single-station receivers as a sixth item with want_uv."""
         nt = self.s.nt if max_steps <= 0 else min(max_steps, self.s.nt)
         nr = len(self.sc.rec_xy)
-        trace = np.zeros(max(nt * nr * 7, 1))
+        trace = np.zeros(max(nt * nr * 7, 1)) if want_trace else None
         times = np.zeros(max(nt, 1))
+        uvt = np.zeros(max(nt * self.nuv * 4, 1)) if want_uv else None
         done = C.c_int(0)
         n2 = 2 * self.nnodes
         u, v, a = (np.zeros(n2) for _ in range(3)) if want_state else (None, None, None)
         init = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u0, v0, a0)]
-        _chk(lib().orc_run(C.byref(self.s), max_steps, *[None if x is None else _dp(x) for x in init], _dp(trace),
-                           _dp(times), C.byref(done), None if u is None else _dp(u), None if v is None else _dp(v),
-                           None if a is None else _dp(a)))
+        _chk(lib().orc_run_ex(C.byref(self.s), max_steps, *[None if x is None else _dp(x) for x in init],
+                              None if trace is None else _dp(trace), _dp(times), C.byref(done),
+                              None if u is None else _dp(u), None if v is None else _dp(v),
+                              None if a is None else _dp(a), None if uvt is None else _dp(uvt)))
         k = done.value
-        return times[:k], trace[: k * nr * 7].reshape(k, nr, 7), u, v, a
+        out = (times[:k], None if trace is None else trace[: k * nr * 7].reshape(k, nr, 7), u, v, a)
+        if want_uv:
+            out = out + (uvt[: k * self.nuv * 4].reshape(k, self.nuv, 4),)
+        return out
+
+    def receiver_rows(self, u, v, a):
+        """Receiver rows [nr][7] (ux uy vx vy ax ay curl) for the given state."""
+        nr = len(self.sc.rec_xy)
+        out = np.zeros(max(nr * 7, 1))
+        arr = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, a)]
+        _chk(lib().orc_receiver_rows(C.byref(self.s), *[_dp(x) for x in arr], _dp(out)))
+        return out[: nr * 7].reshape(nr, 7)
+
+    def uv_rows(self, u=None, v=None):
+        """Single-station rows [nuv][4] for the given state (None: only the
+        stations), with the stations' elements and nodal weights."""
+        out = np.zeros(max(self.nuv * 4, 1))
+        w = np.zeros(max(self.nuv * self.npe, 1))
+        el = np.zeros(max(self.nuv, 1), dtype=np.int32)
+        arr = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u, v)]
+        _chk(lib().orc_uv_rows(C.byref(self.s), *[None if x is None else _dp(x) for x in arr], _dp(out), _dp(w),
+                               _ip(el)))
+        return (out[: self.nuv * 4].reshape(self.nuv, 4), el[: self.nuv], w[: self.nuv * self.npe].reshape(self.nuv, self.npe))
 
     def write_receiver_files(self, directory, times, trace):
         t = np.ascontiguousarray(times, dtype=np.float64)

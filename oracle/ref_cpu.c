@@ -385,6 +385,15 @@ typedef struct {
   int nt;
   double tmax;
   int nthreads;
+  /* Optional fields (zero / NULL keeps the behaviour above). */
+  const int *src_impl;  /* [nsrc]: 1 = nearest GLL node (SOURCE_IMPL_NEAREST_QPOINT, also when NULL),
+                           0 = basis at the source's own local coordinate (SOURCE_IMPL_POINTWISE, EXT) */
+  int nuv;              /* single-station receivers, RecordUV (se2wave.c:2351-2443) and
+                           RecordUV_interp (2445-2581) */
+  const double *uv_xy;  /* [nuv][2] */
+  const int *uv_interp; /* [nuv]: 0 = nearest node of the element, 1 = Lagrange interpolation */
+  const char *rec_dir;  /* non-NULL: every step appends its row to the receiver file in this
+                           directory (fopen/fprintf/fclose per step, 2904-2981) */
 } orc_scenario;
 
 typedef struct {
@@ -685,13 +694,17 @@ typedef struct {
 } orc_source;
 
 /* SeismicSourceCreate 2995-3243 (single rank) + SeismicSourceSetup_Pointwise
- * 3310-3365 + SeismicSourceSetup_Moment_Pointwise 3367-3429.  EXT: force. */
+ * 3310-3365 + SeismicSourceSetup_Moment_Pointwise 3367-3429.  EXT: force;
+ * EXT: impl 0 (SOURCE_IMPL_POINTWISE, which the reference rejects at
+ * 3210-3211) evaluates the basis at the source's local coordinate xi_source
+ * (3097-3104) as SeismicSourceSetup_Pointwise's pointwise branch does
+ * (3339-3342), with no GLL node search. */
 static int orc_source_setup(const orc_mesh *m, const double coor[2], int type, const double *values, int use_internal,
-                            orc_source *s)
+                            int impl, orc_source *s)
 {
   double dx, dy, gmin_d[2], gmax_d[2], xi_src[2];
   int ii, jj, e, i, nj, ni, min_qp = -1, npe = m->npe;
-  double elc[2 * 121];
+  double elc[2 * 121] = {0};
   if (m->p > 10) return orc_fail(ORC_ERR_SUP, "p > 10");
   dx = (m->gmax[0] - m->gmin[0]) / ((double)m->mx);
   dy = (m->gmax[1] - m->gmin[1]) / ((double)m->my);
@@ -706,13 +719,32 @@ static int orc_source_setup(const orc_mesh *m, const double coor[2], int type, c
   e = ii + jj * m->mx;
   xi_src[0] = 2.0 * (coor[0] - (gmin_d[0] + ii * dx)) / dx - 1.0;
   xi_src[1] = 2.0 * (coor[1] - (gmin_d[1] + jj * dy)) / dy - 1.0;
-  (void)xi_src;
-  if ((m->p == 1) && use_internal) return orc_fail(ORC_ERR_USER, "Cannot use basis degree = 1 with nearest interior GLL");
+  if (fabs(xi_src[0]) < 1.0e-12) xi_src[0] = 0.0;
+  if (fabs(xi_src[1]) < 1.0e-12) xi_src[1] = 0.0;
   for (i = 0; i < npe; i++) {
     long nidx = m->element[(size_t)npe * e + i];
     elc[2 * i] = m->coor[2 * nidx];
     elc[2 * i + 1] = m->coor[2 * nidx + 1];
   }
+  if (impl == 0) {
+    double dN_dxi_q[121], dN_deta_q[121], N_q[121], dN_dx_q[121], dN_dy_q[121];
+    s->elem = e;
+    s->closest_gll = -1;
+    orc_tensor2d_at_point(xi_src, m->p, dN_dxi_q, dN_deta_q, N_q);
+    orc_derivs(m, 1, elc, dN_dxi_q, dN_deta_q, dN_dx_q, dN_dy_q);
+    for (i = 0; i < npe; i++) {
+      if (type == 1) {
+        s->vals[2 * i] = (values[0] * dN_dx_q[i] + values[1] * dN_dy_q[i]);
+        s->vals[2 * i + 1] = (values[2] * dN_dx_q[i] + values[3] * dN_dy_q[i]);
+      } else if (type == 0) {
+        s->vals[2 * i] = values[0] * N_q[i];
+        s->vals[2 * i + 1] = values[1] * N_q[i];
+      } else return orc_fail(ORC_ERR_ARG, "unknown source type");
+    }
+    return ORC_OK;
+  }
+  if (impl != 1) return orc_fail(ORC_ERR_SUP, "Source implementation type not supported");
+  if ((m->p == 1) && use_internal) return orc_fail(ORC_ERR_USER, "Cannot use basis degree = 1 with nearest interior GLL");
   {
     double sep2_min = 1.7976931348623157e308;
     int lo = use_internal ? 1 : 0, hi = use_internal ? m->nb - 1 : m->nb;
@@ -803,6 +835,73 @@ static void orc_receiver_sample(const orc_mesh *m, const orc_receiver *r, const 
   out[6] = dvydx - dvxdy;
 }
 
+typedef struct {
+  int interp, eid, nid;
+  double N[121]; /* interp: tensor basis at the station, k = i + j*nb */
+} orc_uv_receiver;
+
+/* Station of RecordUV (se2wave.c:2386-2419: containing element by
+ * truncation, no clamp; nearest node of that element, strict <, first in
+ * element-local order) or RecordUV_interp (2492-2532: containing element,
+ * local coordinates xi = 2(x - x0)/dx - 1, N[k] = N_i(xi) N_j(eta) with i
+ * running fastest). */
+static int orc_uv_locate(const orc_mesh *m, const double xr[2], int interp, orc_uv_receiver *r)
+{
+  double dx, dy;
+  int ei, ej, n;
+  dx = (m->gmax[0] - m->gmin[0]) / ((double)m->mx);
+  ei = (int)((xr[0] - m->gmin[0]) / dx);
+  dy = (m->gmax[1] - m->gmin[1]) / ((double)m->my);
+  ej = (int)((xr[1] - m->gmin[1]) / dy);
+  if (xr[0] < m->gmin[0] || xr[1] < m->gmin[1] || ei < 0 || ej < 0 || ei >= m->mx || ej >= m->my)
+    return orc_fail(ORC_ERR_USER, "receiver has no containing element");
+  memset(r, 0, sizeof(*r));
+  r->interp = interp;
+  r->eid = ei + ej * m->mx;
+  r->nid = -1;
+  if (!interp) {
+    double sep2min = 1.0e32;
+    for (n = 0; n < m->npe; n++) {
+      long b = m->element[(size_t)m->npe * r->eid + n];
+      double sep2 = (xr[0] - m->coor[2 * b]) * (xr[0] - m->coor[2 * b]);
+      sep2 += (xr[1] - m->coor[2 * b + 1]) * (xr[1] - m->coor[2 * b + 1]);
+      if (sep2 < sep2min) { r->nid = (int)b; sep2min = sep2; }
+    }
+  } else {
+    double x0 = m->gmin[0] + ei * dx, y0 = m->gmin[1] + ej * dy, xi, eta, Ns1[11], Ns2[11];
+    int i, j, k = 0;
+    xi = 2.0 * (xr[0] - x0) / dx - 1.0;
+    eta = 2.0 * (xr[1] - y0) / dy - 1.0;
+    orc_basis1d(m->p, 1, &xi, Ns1);
+    orc_basis1d(m->p, 1, &eta, Ns2);
+    for (j = 0; j < m->nb; j++)
+      for (i = 0; i < m->nb; i++) r->N[k++] = Ns1[i] * Ns2[j];
+  }
+  return ORC_OK;
+}
+
+/* One RecordUV / RecordUV_interp row (ux uy vx vy): the nearest node's values
+ * (2436) or ur += N[k] u_k, vr += N[k] v_k over the element's nodes in
+ * ascending local order (2560-2571). */
+static void orc_uv_sample(const orc_mesh *m, const orc_uv_receiver *r, const double *u, const double *v, double *out)
+{
+  if (!r->interp) {
+    out[0] = u[2 * (long)r->nid]; out[1] = u[2 * (long)r->nid + 1];
+    out[2] = v[2 * (long)r->nid]; out[3] = v[2 * (long)r->nid + 1];
+  } else {
+    double ur0 = 0.0, ur1 = 0.0, vr0 = 0.0, vr1 = 0.0;
+    int k;
+    for (k = 0; k < m->npe; k++) {
+      long nid = m->element[(size_t)m->npe * r->eid + k];
+      ur0 += r->N[k] * u[2 * nid + 0];
+      ur1 += r->N[k] * u[2 * nid + 1];
+      vr0 += r->N[k] * v[2 * nid + 0];
+      vr1 += r->N[k] * v[2 * nid + 1];
+    }
+    out[0] = ur0; out[1] = ur1; out[2] = vr0; out[3] = vr1;
+  }
+}
+
 /* ------------------------------------------------------------------------ */
 /* Exported entry points (ctypes)                                            */
 /* ------------------------------------------------------------------------ */
@@ -852,13 +951,16 @@ int orc_compute_timestep(const orc_scenario *sc, double *dt)
   return ORC_OK;
 }
 
+static int orc_src_impl(const orc_scenario *sc, int s) { return sc->src_impl ? sc->src_impl[s] : 1; }
+
 int orc_source_values(const orc_scenario *sc, int s, int *elem, int *closest_gll, double *vals)
 {
   orc_mesh m;
   orc_source src;
   int ierr = orc_mesh_create(sc, &m);
   if (ierr) return ierr;
-  ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll, &src);
+  ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll,
+                          orc_src_impl(sc, s), &src);
   if (!ierr) {
     *elem = src.elem;
     *closest_gll = src.closest_gll;
@@ -883,19 +985,117 @@ int orc_receiver_nodes(const orc_scenario *sc, int *nid, int *eid, int *gll)
   return ierr;
 }
 
+/* Receiver rows (ux uy vx vy ax ay curl, se2wave.c:2908-2959) of the scenario's
+ * nrec stations for a given state u, v, a: out[r][7]. */
+int orc_receiver_rows(const orc_scenario *sc, const double *u, const double *v, const double *a, double *out)
+{
+  orc_mesh m;
+  int r, ierr = orc_mesh_create(sc, &m);
+  if (ierr) return ierr;
+  for (r = 0; r < sc->nrec; r++) {
+    orc_receiver rc;
+    ierr = orc_receiver_locate(&m, &sc->rec_xy[2 * r], &rc);
+    if (ierr) break;
+    orc_receiver_sample(&m, &rc, u, v, a, &out[(size_t)r * 7]);
+  }
+  orc_mesh_free(&m);
+  return ierr;
+}
+
+/* Single-station rows (ux uy vx vy) of the scenario's nuv RecordUV /
+ * RecordUV_interp stations for a given state u, v: out[r][4]; weights[r][npe]
+ * (may be NULL) receives each station's nodal weights (the basis values, or 1
+ * at the nearest node) and elem[r] (may be NULL) its element. */
+int orc_uv_rows(const orc_scenario *sc, const double *u, const double *v, double *out, double *weights, int *elem)
+{
+  orc_mesh m;
+  int r, k, ierr = orc_mesh_create(sc, &m);
+  if (ierr) return ierr;
+  for (r = 0; r < sc->nuv; r++) {
+    orc_uv_receiver rc;
+    ierr = orc_uv_locate(&m, &sc->uv_xy[2 * r], sc->uv_interp ? sc->uv_interp[r] : 0, &rc);
+    if (ierr) break;
+    if (u && v && out) orc_uv_sample(&m, &rc, u, v, &out[(size_t)r * 4]);
+    if (elem) elem[r] = rc.eid;
+    if (weights) {
+      for (k = 0; k < m.npe; k++) {
+        double w = rc.N[k];
+        if (!rc.interp) w = (m.element[(size_t)m.npe * rc.eid + k] == rc.nid) ? 1.0 : 0.0;
+        weights[(size_t)r * m.npe + k] = w;
+      }
+    }
+  }
+  orc_mesh_free(&m);
+  return ierr;
+}
+
+/* Header of the receiver file of RecordUVA_MultipleStations_NearestGLL_MPI on
+ * one rank (2808-2842). */
+static int orc_receiver_file_header(const orc_scenario *sc, const orc_mesh *m, const orc_receiver *recs, const char *dir,
+                                    char *fname, size_t fname_len)
+{
+  FILE *fp;
+  int r;
+  snprintf(fname, fname_len, "%s/closestqpsource-receiverCP-uva-%dx%d-p%d-rank%d.dat", dir, m->mx, m->my, m->p, 0);
+  fp = fopen(fname, "w");
+  if (!fp) return orc_fail(ORC_ERR_FILE, "cannot open receiver file");
+  fprintf(fp, "# SpecFECtx meta data\n");
+  fprintf(fp, "#   mx %d : my %d : basis order %d\n", m->mx, m->my, m->p);
+  fprintf(fp, "# Receiver meta data\n");
+  fprintf(fp, "#   + number receiver locations: %d\n", sc->nrec);
+  fprintf(fp, "#   + number receiver locations <local>: %d\n", sc->nrec);
+  fprintf(fp, "#   + takes displ/velo/accel from basis nearest to requested receiver location\n");
+  for (r = 0; r < sc->nrec; r++) {
+    fprintf(fp, "#   + receiver location [%d]: x,y %+1.8e %+1.8e\n", r, sc->rec_xy[2 * r], sc->rec_xy[2 * r + 1]);
+    fprintf(fp, "#   +   mapped to nearest node --> %+1.8e %+1.8e\n", m->coor[2 * (long)recs[r].nid], m->coor[2 * (long)recs[r].nid + 1]);
+  }
+  if (sc->nrec != 0) {
+    fprintf(fp, "# Time series header <field>(<column index>)\n");
+    fprintf(fp, "#   time(1)\n");
+    for (r = 0; r < sc->nrec; r++) {
+      int offset = 1 + r * 7;
+      fprintf(fp, "#     ux(%d) uy(%d) vx(%d) vy(%d) ax(%d) ay(%d) curl(v) (%d)-> station [%d]\n", offset + 1, offset + 2,
+              offset + 3, offset + 4, offset + 5, offset + 6, offset + 7, r);
+    }
+  } else {
+    fprintf(fp, "# <note> No receivers found on this sub-domain\n");
+    fprintf(fp, "# <note> This file will remain empty\n");
+  }
+  fclose(fp);
+  return ORC_OK;
+}
+
+/* One row of that file (2904-2962): "%1.4e", then " %+1.8e" x 7 per station. */
+static void orc_receiver_file_row(FILE *fp, int nrec, double time, const double *row)
+{
+  int r;
+  fprintf(fp, "%1.4e", time);
+  for (r = 0; r < nrec; r++) {
+    const double *o = &row[(size_t)r * 7];
+    fprintf(fp, " %+1.8e %+1.8e %+1.8e %+1.8e %+1.8e %+1.8e", o[0], o[1], o[2], o[3], o[4], o[5]);
+    fprintf(fp, " %+1.8e", o[6]);
+  }
+  fprintf(fp, "\n");
+}
+
 /* se2wave_demo time loop, se2wave.c:4722-4775, with the receiver sampling of
  * 2904-2962.  trace[k][r][7] and times[k] are written for every completed
  * step; *nsteps_done receives the number of steps taken.  u, v, a (may be
  * NULL) receive the final state.  If u0/v0/a0 are non-NULL they are the
  * initial state (the reference starts from zero).  max_steps > 0 truncates
- * the loop (timing samples). */
-int orc_run(const orc_scenario *sc, int max_steps, const double *u0, const double *v0, const double *a0, double *trace,
-            double *times, int *nsteps_done, double *u_out, double *v_out, double *a_out)
+ * the loop (timing samples).  uv_trace[k][r][4] (may be NULL) receives the
+ * rows of the scenario's RecordUV / RecordUV_interp stations.  With
+ * sc->rec_dir set, every step appends its row to the receiver file there
+ * (open, append, close: the reference's per-step file I/O, 2904-2981). */
+int orc_run_ex(const orc_scenario *sc, int max_steps, const double *u0, const double *v0, const double *a0, double *trace,
+               double *times, int *nsteps_done, double *u_out, double *v_out, double *a_out, double *uv_trace)
 {
   orc_mesh m;
   orc_source *srcs = NULL;
   orc_receiver *recs = NULL;
-  double *u, *v, *a, *f, *g, *Md, *febuf, time = 0.0, dt = sc->dt;
+  orc_uv_receiver *uvs = NULL;
+  double *u, *v, *a, *f, *g, *Md, *febuf, *row = NULL, time = 0.0, dt = sc->dt;
+  char fname[1024];
   long n2, i;
   int k, s, r, ierr, nt = sc->nt;
   ierr = orc_mesh_create(sc, &m);
@@ -908,14 +1108,16 @@ int orc_run(const orc_scenario *sc, int max_steps, const double *u0, const doubl
   g = (double *)calloc(n2, sizeof(double));
   Md = (double *)calloc(n2, sizeof(double));
   febuf = (double *)malloc(sizeof(double) * 2 * (size_t)m.npe * m.ne);
-  if (!u || !v || !a || !f || !g || !Md || !febuf) { ierr = orc_fail(ORC_ERR_MEM, "out of memory"); goto done; }
+  row = (double *)malloc(sizeof(double) * 7 * (size_t)(sc->nrec > 0 ? sc->nrec : 1));
+  if (!u || !v || !a || !f || !g || !Md || !febuf || !row) { ierr = orc_fail(ORC_ERR_MEM, "out of memory"); goto done; }
   if (u0) memcpy(u, u0, sizeof(double) * n2);
   if (v0) memcpy(v, v0, sizeof(double) * n2);
   if (a0) memcpy(a, a0, sizeof(double) * n2);
   orc_mass(&m, Md);
   srcs = (orc_source *)calloc(sc->nsrc > 0 ? sc->nsrc : 1, sizeof(orc_source));
   for (s = 0; s < sc->nsrc; s++) {
-    ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll, &srcs[s]);
+    ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll,
+                            orc_src_impl(sc, s), &srcs[s]);
     if (ierr) goto done;
   }
   recs = (orc_receiver *)calloc(sc->nrec > 0 ? sc->nrec : 1, sizeof(orc_receiver));
@@ -923,6 +1125,12 @@ int orc_run(const orc_scenario *sc, int max_steps, const double *u0, const doubl
     ierr = orc_receiver_locate(&m, &sc->rec_xy[2 * r], &recs[r]);
     if (ierr) goto done;
   }
+  uvs = (orc_uv_receiver *)calloc(sc->nuv > 0 ? sc->nuv : 1, sizeof(orc_uv_receiver));
+  for (r = 0; r < sc->nuv; r++) {
+    ierr = orc_uv_locate(&m, &sc->uv_xy[2 * r], sc->uv_interp ? sc->uv_interp[r] : 0, &uvs[r]);
+    if (ierr) goto done;
+  }
+  if (sc->rec_dir && (ierr = orc_receiver_file_header(sc, &m, recs, sc->rec_dir, fname, sizeof(fname)))) goto done;
   if (max_steps > 0 && max_steps < nt) nt = max_steps;
   for (k = 1; k <= nt; k++) {
     time = time + dt;
@@ -946,8 +1154,18 @@ int orc_run(const orc_scenario *sc, int max_steps, const double *u0, const doubl
     for (i = 0; i < n2; i++) a[i] = (Md[i] != 0.0) ? f[i] / Md[i] : 0.0;
     for (i = 0; i < n2; i++) v[i] = v[i] + (0.5 * dt) * a[i];
     if (times) times[k - 1] = time;
-    if (trace) {
-      for (r = 0; r < sc->nrec; r++) orc_receiver_sample(&m, &recs[r], u, v, a, &trace[((size_t)(k - 1) * sc->nrec + r) * 7]);
+    if (trace || sc->rec_dir) {
+      double *o = trace ? &trace[(size_t)(k - 1) * sc->nrec * 7] : row;
+      for (r = 0; r < sc->nrec; r++) orc_receiver_sample(&m, &recs[r], u, v, a, &o[(size_t)r * 7]);
+      if (sc->rec_dir && sc->nrec > 0) {
+        FILE *fp = fopen(fname, "a");
+        if (!fp) { ierr = orc_fail(ORC_ERR_FILE, "cannot open receiver file"); goto done; }
+        orc_receiver_file_row(fp, sc->nrec, time, o);
+        fclose(fp);
+      }
+    }
+    if (uv_trace) {
+      for (r = 0; r < sc->nuv; r++) orc_uv_sample(&m, &uvs[r], u, v, &uv_trace[((size_t)(k - 1) * sc->nuv + r) * 4]);
     }
     if (nsteps_done) *nsteps_done = k;
     if (time >= sc->tmax) break;
@@ -957,9 +1175,15 @@ int orc_run(const orc_scenario *sc, int max_steps, const double *u0, const doubl
   if (a_out) memcpy(a_out, a, sizeof(double) * n2);
   ierr = ORC_OK;
 done:
-  free(u); free(v); free(a); free(f); free(g); free(Md); free(febuf); free(srcs); free(recs);
+  free(u); free(v); free(a); free(f); free(g); free(Md); free(febuf); free(row); free(srcs); free(recs); free(uvs);
   orc_mesh_free(&m);
   return ierr;
+}
+
+int orc_run(const orc_scenario *sc, int max_steps, const double *u0, const double *v0, const double *a0, double *trace,
+            double *times, int *nsteps_done, double *u_out, double *v_out, double *a_out)
+{
+  return orc_run_ex(sc, max_steps, u0, v0, a0, trace, times, nsteps_done, u_out, v_out, a_out, NULL);
 }
 
 /* Writes the receiver file of RecordUVA_MultipleStations_NearestGLL_MPI on one
@@ -978,42 +1202,11 @@ int orc_write_receiver_files(const orc_scenario *sc, const char *dir, int nsteps
     ierr = orc_receiver_locate(&m, &sc->rec_xy[2 * r], &recs[r]);
     if (ierr) { free(recs); orc_mesh_free(&m); return ierr; }
   }
-  snprintf(fname, sizeof(fname), "%s/closestqpsource-receiverCP-uva-%dx%d-p%d-rank%d.dat", dir, m.mx, m.my, m.p, 0);
-  fp = fopen(fname, "w");
+  if ((ierr = orc_receiver_file_header(sc, &m, recs, dir, fname, sizeof(fname)))) { free(recs); orc_mesh_free(&m); return ierr; }
+  fp = fopen(fname, "a");
   if (!fp) { free(recs); orc_mesh_free(&m); return orc_fail(ORC_ERR_FILE, "cannot open receiver file"); }
-  fprintf(fp, "# SpecFECtx meta data\n");
-  fprintf(fp, "#   mx %d : my %d : basis order %d\n", m.mx, m.my, m.p);
-  fprintf(fp, "# Receiver meta data\n");
-  fprintf(fp, "#   + number receiver locations: %d\n", sc->nrec);
-  fprintf(fp, "#   + number receiver locations <local>: %d\n", sc->nrec);
-  fprintf(fp, "#   + takes displ/velo/accel from basis nearest to requested receiver location\n");
-  for (r = 0; r < sc->nrec; r++) {
-    fprintf(fp, "#   + receiver location [%d]: x,y %+1.8e %+1.8e\n", r, sc->rec_xy[2 * r], sc->rec_xy[2 * r + 1]);
-    fprintf(fp, "#   +   mapped to nearest node --> %+1.8e %+1.8e\n", m.coor[2 * (long)recs[r].nid], m.coor[2 * (long)recs[r].nid + 1]);
-  }
-  if (sc->nrec != 0) {
-    fprintf(fp, "# Time series header <field>(<column index>)\n");
-    fprintf(fp, "#   time(1)\n");
-    for (r = 0; r < sc->nrec; r++) {
-      int offset = 1 + r * 7;
-      fprintf(fp, "#     ux(%d) uy(%d) vx(%d) vy(%d) ax(%d) ay(%d) curl(v) (%d)-> station [%d]\n", offset + 1, offset + 2,
-              offset + 3, offset + 4, offset + 5, offset + 6, offset + 7, r);
-    }
-  } else {
-    fprintf(fp, "# <note> No receivers found on this sub-domain\n");
-    fprintf(fp, "# <note> This file will remain empty\n");
-  }
-  if (sc->nrec != 0) {
-    for (k = 0; k < nsteps; k++) {
-      fprintf(fp, "%1.4e", times[k]);
-      for (r = 0; r < sc->nrec; r++) {
-        const double *o = &trace[((size_t)k * sc->nrec + r) * 7];
-        fprintf(fp, " %+1.8e %+1.8e %+1.8e %+1.8e %+1.8e %+1.8e", o[0], o[1], o[2], o[3], o[4], o[5]);
-        fprintf(fp, " %+1.8e", o[6]);
-      }
-      fprintf(fp, "\n");
-    }
-  }
+  if (sc->nrec != 0)
+    for (k = 0; k < nsteps; k++) orc_receiver_file_row(fp, sc->nrec, times[k], &trace[(size_t)k * sc->nrec * 7]);
   fclose(fp);
   snprintf(fname, sizeof(fname), "%s/closestqpsource-receiverCP-uva-%dx%d-p%d.mpimeta", dir, m.mx, m.my, m.p);
   fp = fopen(fname, "w");

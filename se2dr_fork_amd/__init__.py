@@ -75,6 +75,9 @@ def _declare(hip, host):
     hip.se2w_snapshot_begin.argtypes = [v]
     hip.se2w_snapshot_wait.argtypes = [v, C.POINTER(_dp), C.POINTER(_dp)]
     hip.se2w_run.argtypes = [v, i32, d, _dp]
+    hip.se2w_reserve.argtypes = [v, i32]
+    hip.se2w_get_reserved.argtypes = [v, C.POINTER(C.c_int64)]
+    hip.se2w_get_counts.argtypes = [v, _ip]
     hip.se2w_get_receivers.argtypes = [v, _dp]
     hip.se2w_assemble_forces.argtypes = [v, _dp, _dp]
     hip.se2w_norms.argtypes = [v, _dp]
@@ -156,6 +159,7 @@ def _declare(hip, host):
     host.se2w_solver_context.argtypes = [v]
     host.se2w_solver_context.restype = v
     host.se2w_solver_advance.argtypes = [v, C.c_int, d]
+    host.se2w_solver_reserve.argtypes = [v, C.c_int]
     host.se2w_solver_time.argtypes = [v]
     host.se2w_solver_time.restype = d
     host.se2w_solver_get_trace.argtypes = [v, _dp, _dp]
@@ -449,6 +453,11 @@ class Solver:
         self.ctx = Context(_handle=self._host.se2w_solver_context(self.h), spec=spec, precision=precision)
         self.nr = receivers.nr if receivers else 0
 
+    def reserve(self, nsteps: int):
+        """Sizes the buffers of advances of up to nsteps steps ahead of them,
+        so that such an advance allocates nothing."""
+        _hchk(self._host.se2w_solver_reserve(self.h, nsteps))
+
     def advance(self, nsteps: int, dt: float):
         _hchk(self._host.se2w_solver_advance(self.h, nsteps, dt))
         times = np.zeros(nsteps)
@@ -521,30 +530,47 @@ class Context:
         except Exception:
             pass
 
+    def _sized(self, name, a, n, dtype=np.float64):
+        """a as a contiguous array of exactly n values (ValueError otherwise):
+        the library reads or writes that many through the pointer."""
+        a = np.ascontiguousarray(a, dtype=dtype).reshape(-1)
+        if a.size != n:
+            raise ValueError(f"{name}: expected {n} values, got {a.size}")
+        return a
+
+    def counts(self):
+        """(sources, receivers, element receivers, steps of the last run)."""
+        c = np.zeros(4, dtype=np.int32)
+        _gchk(self._hip.se2w_get_counts(self._handle, _ptr(c, _ip)))
+        return tuple(int(x) for x in c)
+
     def set_tables(self, D, w1d, elem_dx, elem_dy):
-        a = [np.ascontiguousarray(x, dtype=np.float64).reshape(-1) for x in (D, w1d, elem_dx, elem_dy)]
+        nb = self.p + 1
+        a = [self._sized("D", D, nb * nb), self._sized("w1d", w1d, nb), self._sized("elem_dx", elem_dx, self.mx),
+             self._sized("elem_dy", elem_dy, self.my)]
         _gchk(self._hip.se2w_set_tables(self._handle, *[_ptr(x) for x in a]))
 
     def set_cells(self, lam, mu):
-        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (lam, mu)]
+        ne = self.mx * self.my
+        a = [self._sized("lambda", lam, ne), self._sized("mu", mu, ne)]
         _gchk(self._hip.se2w_set_cells(self._handle, _ptr(a[0]), _ptr(a[1])))
 
     def set_mass(self, m):
-        m = np.ascontiguousarray(m, dtype=np.float64)
+        m = self._sized("mass", m, self.nnodes)
         _gchk(self._hip.se2w_set_mass(self._handle, _ptr(m)))
 
     def set_sources(self, elem, vals):
-        e = np.ascontiguousarray(elem, dtype=np.int32)
-        v = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
+        e = np.ascontiguousarray(elem, dtype=np.int32).reshape(-1)
+        v = self._sized("vals", vals, len(e) * 2 * self.npe)
         _gchk(self._hip.se2w_set_sources(self._handle, len(e), _ptr(e, _ip), _ptr(v)))
 
     def set_receivers(self, elem, gll_idx):
-        e = np.ascontiguousarray(elem, dtype=np.int32)
-        g = np.ascontiguousarray(gll_idx, dtype=np.int32)
+        e = np.ascontiguousarray(elem, dtype=np.int32).reshape(-1)
+        g = self._sized("gll", gll_idx, len(e), np.int32)
         _gchk(self._hip.se2w_set_receivers(self._handle, len(e), _ptr(e, _ip), _ptr(g, _ip)))
 
     def set_state(self, u=None, v=None, a=None):
-        arrs = [None if x is None else np.ascontiguousarray(x, dtype=np.float64) for x in (u, v, a)]
+        arrs = [None if x is None else self._sized(nm, x, 2 * self.nnodes) for nm, x in (("u", u), ("v", v), ("a", a))]
         _gchk(self._hip.se2w_set_state(self._handle, *[None if x is None else _ptr(x) for x in arrs]))
 
     def get_state(self):
@@ -563,17 +589,41 @@ class Context:
         n = 2 * self.nnodes
         return np.ctypeslib.as_array(pu, shape=(n,)).copy(), np.ctypeslib.as_array(pv, shape=(n,)).copy()
 
+    def reserve(self, nsteps: int):
+        """Sizes the run buffers for runs of up to nsteps steps (se2w_reserve)."""
+        _gchk(self._hip.se2w_reserve(self._handle, nsteps))
+
+    def reserved(self):
+        """(psi values, receiver record values, element-receiver record values, pinned bytes)."""
+        c = np.zeros(4, dtype=np.int64)
+        _gchk(self._hip.se2w_get_reserved(self._handle, c.ctypes.data_as(C.POINTER(C.c_int64))))
+        return tuple(int(x) for x in c)
+
     def run(self, nsteps: int, dt: float, psi=None):
-        p = None if psi is None else _ptr(np.ascontiguousarray(psi, dtype=np.float64))
+        ns = self.counts()[0]
+        p = None
+        if ns > 0 or psi is not None:
+            if psi is None:
+                raise ValueError(f"psi: a table of {nsteps} x {ns} values is required (sources are set)")
+            a = np.ascontiguousarray(psi, dtype=np.float64).reshape(-1)
+            if a.size < nsteps * ns:
+                raise ValueError(f"psi: expected {nsteps * ns} values, got {a.size}")
+            p = _ptr(a)
         _gchk(self._hip.se2w_run(self._handle, nsteps, dt, p))
 
-    def receivers(self, nsteps: int, nr: int) -> np.ndarray:
-        out = np.zeros(max(nsteps * nr * 7, 1))
+    def receivers(self, nsteps: Optional[int] = None, nr: Optional[int] = None) -> np.ndarray:
+        """Receiver rows [steps][nr][7] of the last run, sized from the
+        context's own receiver count and run length (the arguments, if given,
+        must agree with them)."""
+        _, n_r, _, steps = self.counts()
+        if (nsteps is not None and nsteps != steps) or (nr is not None and nr != n_r):
+            raise ValueError(f"the last run has {steps} steps and {n_r} receivers")
+        out = np.zeros(max(steps * n_r * 7, 1))
         _gchk(self._hip.se2w_get_receivers(self._handle, _ptr(out)))
-        return out[: nsteps * nr * 7].reshape(nsteps, nr, 7)
+        return out[: steps * n_r * 7].reshape(steps, n_r, 7)
 
     def assemble_forces(self, u) -> np.ndarray:
-        u = np.ascontiguousarray(u, dtype=np.float64)
+        u = self._sized("u", u, 2 * self.nnodes)
         f = np.zeros_like(u)
         _gchk(self._hip.se2w_assemble_forces(self._handle, _ptr(u), _ptr(f)))
         return f

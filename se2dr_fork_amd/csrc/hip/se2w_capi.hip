@@ -17,6 +17,7 @@
 #include "se2w_kernels_reg.hpp"
 #include "se2w_kernels_mfma.hpp"
 #include "se2w_ens.hpp"
+#include "se2w_launch.hpp"
 
 using namespace se2w;
 
@@ -334,104 +335,16 @@ bool reg_variant_built(int p, int variant, int schedule) {
   return is_reg_variant(variant) && reg_schedule_built(p, reg_schedule_of(variant), schedule == SE2W_SCHED_FUSED);
 }
 
-template <int P, int MODE, bool FUSED, int S>
-bool launch_reg_s(se2w_ctx *c, const StepParams<float> &prm) {
-  if constexpr (S < kNumRegSchedules) {
-    if (reg_schedule_built(P, S, FUSED) && reg_schedule_of(c->variant) == S) {
-      if constexpr (reg_schedule_built(P, S, FUSED)) {
-        constexpr RegSchedule r = kRegSchedules[S];
-        hipLaunchKernelGGL((reg_kernel<P, MODE, FUSED, r.ex, r.ey, r.waves, r.pf, r.lpe>), dim3(8 * c->tiles_per_xcd),
-                           dim3(r.ex * r.ey * r.lpe), 0, c->stream, prm);
-        return true;
-      }
-    }
-    return launch_reg_s<P, MODE, FUSED, S + 1>(c, prm);
-  }
-  return false;
-}
-
-// Element (+ update when fused) kernel of the context's variant.
-template <int P, int MODE, bool FUSED>
-void launch_element_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
-  const dim3 grid(8 * c->tiles_per_xcd);
-  if constexpr (MODE != MODE_F64 && P >= 2) {
-    if (is_reg_variant(c->variant) && launch_reg_s<P, MODE, FUSED, 0>(c, prm)) return;
-  }
-  if constexpr (MODE != MODE_F64 && P >= 6 && FUSED) {
-    if (c->variant == SE2W_KERNEL_MFMA) {
-      hipLaunchKernelGGL((mfma_kernel<P, MODE>), grid, dim3(kThreads), 0, c->stream, prm);
-      return;
-    }
-    if (c->variant == SE2W_KERNEL_LINE) {
-      hipLaunchKernelGGL((line_kernel<P, MODE>), grid, dim3(kThreads), 0, c->stream, prm);
-      return;
-    }
-  }
-  if constexpr (FUSED) {
-    if (c->variant == SE2W_KERNEL_LDS_WIDE) {
-      hipLaunchKernelGGL((tile_kernel<P, MODE, true, 512>), grid, dim3(512), 0, c->stream, prm);
-      return;
-    }
-  }
-  hipLaunchKernelGGL((tile_kernel<P, MODE, FUSED>), grid, dim3(kThreads), 0, c->stream, prm);
-}
-
-template <int P, int MODE>
-void launch_step_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm, hipEvent_t *evs) {
-  const dim3 grid(8 * c->tiles_per_xcd), block(kThreads);
-  if (evs) hipEventRecord(evs[0], c->stream);
-  if (c->schedule == SE2W_SCHED_FUSED) {
-    launch_element_p<P, MODE, true>(c, prm);
-    if (evs) hipEventRecord(evs[1], c->stream);
-  } else {
-    launch_element_p<P, MODE, false>(c, prm);
-    if (evs) hipEventRecord(evs[1], c->stream);
-    hipLaunchKernelGGL((update_kernel<P, MODE>), grid, block, 0, c->stream, prm);
-    if (evs) hipEventRecord(evs[2], c->stream);
-  }
-}
-
-template <int MODE>
-void launch_step(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm, hipEvent_t *evs) {
-  switch (c->p) {
-    case 1: launch_step_p<1, MODE>(c, prm, evs); break;
-    case 2: launch_step_p<2, MODE>(c, prm, evs); break;
-    case 3: launch_step_p<3, MODE>(c, prm, evs); break;
-    case 4: launch_step_p<4, MODE>(c, prm, evs); break;
-    case 5: launch_step_p<5, MODE>(c, prm, evs); break;
-    case 6: launch_step_p<6, MODE>(c, prm, evs); break;
-    case 7: launch_step_p<7, MODE>(c, prm, evs); break;
-    case 8: launch_step_p<8, MODE>(c, prm, evs); break;
-    default: break;
-  }
-}
-
-template <int P, int MODE>
-void launch_element_only_p(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
-  launch_element_p<P, MODE, false>(c, prm);
-}
-
-template <int MODE>
-void launch_element_only(se2w_ctx *c, const StepParams<typename RealOf<MODE>::T> &prm) {
-  switch (c->p) {
-    case 1: launch_element_only_p<1, MODE>(c, prm); break;
-    case 2: launch_element_only_p<2, MODE>(c, prm); break;
-    case 3: launch_element_only_p<3, MODE>(c, prm); break;
-    case 4: launch_element_only_p<4, MODE>(c, prm); break;
-    case 5: launch_element_only_p<5, MODE>(c, prm); break;
-    case 6: launch_element_only_p<6, MODE>(c, prm); break;
-    case 7: launch_element_only_p<7, MODE>(c, prm); break;
-    case 8: launch_element_only_p<8, MODE>(c, prm); break;
-    default: break;
-  }
-}
+// Launch parameters of a context's step kernels.
+LaunchCfg launch_cfg(const se2w_ctx *c) { return LaunchCfg{c->p, c->variant, c->schedule, 8 * c->tiles_per_xcd, c->stream}; }
 
 int grid_for(long n) { return (int)std::min<long>((n + 255) / 256, 4096); }
 
-// Pinned host buffer for the receiver-record download of the current run.
-// Grown while the run's kernels execute, so pinning overlaps GPU work.
-int ensure_stage(se2w_ctx *c) {
-  const size_t bytes = (size_t)c->rec_steps * c->nr * rec_stride(c->p) * real_size(c->mode);
+// Pinned host buffer for the receiver-record download of a run of nsteps
+// steps.  Grown while the run's kernels execute, so pinning overlaps GPU
+// work (se2w_reserve sizes it ahead of a run instead).
+int ensure_stage_for(se2w_ctx *c, int nsteps) {
+  const size_t bytes = (size_t)nsteps * c->nr * rec_stride(c->p) * real_size(c->mode);
   if (bytes <= c->stage_cap) return SE2W_OK;
   if (c->h_stage) HIPCHK(hipHostFree(c->h_stage));
   c->h_stage = nullptr;
@@ -441,6 +354,59 @@ int ensure_stage(se2w_ctx *c) {
   return SE2W_OK;
 }
 
+int ensure_stage(se2w_ctx *c) { return ensure_stage_for(c, c->rec_steps); }
+
+// Device buffers of a run of nsteps steps: the source-time table, the
+// receiver record and the element-receiver record, grown (never shrunk) to
+// the run's length.
+int ensure_run_buffers(se2w_ctx *c, int nsteps) {
+  const size_t rs = real_size(c->mode);
+  int ierr;
+  if (c->ns > 0) {
+    const size_t n = (size_t)nsteps * c->ns;
+    if (n > c->psi_cap) {
+      dev_free(c->d_psi);
+      c->psi_cap = 0;
+      if ((ierr = dev_alloc(&c->d_psi, n * rs))) return ierr;
+      c->psi_cap = n;
+    }
+  }
+  if (c->nr > 0) {
+    const size_t n = (size_t)nsteps * c->nr * rec_stride(c->p);
+    if (n > c->rec_cap) {
+      dev_free(c->d_rec);
+      c->rec_cap = 0;
+      if ((ierr = dev_alloc(&c->d_rec, n * rs))) return ierr;
+      c->rec_cap = n;
+    }
+  }
+  if (c->ner > 0) {
+    const size_t n = (size_t)nsteps * c->ner * c->npe * 4;
+    if (n > c->erec_cap) {
+      dev_free(c->d_erec);
+      c->erec_cap = 0;
+      if ((ierr = dev_alloc(&c->d_erec, n * rs))) return ierr;
+      c->erec_cap = n;
+    }
+  }
+  return SE2W_OK;
+}
+
+// Events of one call, destroyed on every return path.
+struct ScopedEvents {
+  std::vector<hipEvent_t> ev;
+  ~ScopedEvents() {
+    for (hipEvent_t e : ev)
+      if (e) hipEventDestroy(e);
+  }
+  int create(size_t n) {
+    ev.assign(n, nullptr);
+    for (auto &e : ev) HIPCHK(hipEventCreate(&e));
+    return SE2W_OK;
+  }
+  hipEvent_t &operator[](size_t i) { return ev[i]; }
+};
+
 template <int MODE>
 int run_mode(se2w_ctx *c, int nsteps, double dt) {
   using R = typename RealOf<MODE>::T;
@@ -448,23 +414,20 @@ int run_mode(se2w_ctx *c, int nsteps, double dt) {
   // launches (one event pair, so no event sits between two launches); the
   // split schedule brackets each of its two kernels.
   const bool fused = c->schedule == SE2W_SCHED_FUSED;
-  std::vector<hipEvent_t> evs;
-  if (c->profiling) {
-    evs.resize(fused ? 2 : (size_t)nsteps * 3);
-    for (auto &e : evs) HIPCHK(hipEventCreate(&e));
-  }
+  ScopedEvents evs;
+  if (c->profiling)
+    if (int ierr = evs.create(fused ? 2 : (size_t)nsteps * 3)) return ierr;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
   {
     StepParams<R> prm = make_params<MODE>(c, 0, dt, 0);
-    const long n = (long)c->pitch * c->ny;
-    hipLaunchKernelGGL((predictor_kernel<MODE>), dim3(grid_for(n)), dim3(kThreads), 0, c->stream, prm);
+    launch_predictor<MODE>(prm, (long)c->pitch * c->ny, c->stream);
     HIPCHK(hipGetLastError());
     c->cur ^= 1;
   }
   if (c->profiling && fused) HIPCHK(hipEventRecord(evs[0], c->stream));
   for (int k = 0; k < nsteps; k++) {
     StepParams<R> prm = make_params<MODE>(c, k, dt, k == nsteps - 1);
-    launch_step<MODE>(c, prm, (c->profiling && !fused) ? &evs[(size_t)k * 3] : nullptr);
+    launch_step<MODE>(launch_cfg(c), prm, (c->profiling && !fused) ? &evs[(size_t)k * 3] : nullptr);
     c->cur ^= 1;
   }
   if (c->profiling && fused) HIPCHK(hipEventRecord(evs[1], c->stream));
@@ -493,7 +456,6 @@ int run_mode(se2w_ctx *c, int nsteps, double dt) {
     }
     c->kern_ms[0] = s0 / nsteps;
     c->kern_ms[1] = s1 / nsteps;
-    for (auto &e : evs) hipEventDestroy(e);
   }
   return SE2W_OK;
 }
@@ -505,6 +467,26 @@ namespace se2w {
 int set_error(int code, const char *msg) {
   snprintf(g_err, sizeof(g_err), "%s", msg);
   return code;
+}
+
+static int ens_dispatch_mode(int p, int mode, int variant, const void *args, int grid, hipStream_t s, int *bpc,
+                             int *thr) {
+  switch (mode) {
+    case SE2W_FP64: return ens_dispatch<MODE_F64>(p, variant, args, grid, s, bpc, thr);
+    case SE2W_FP32: return ens_dispatch<MODE_F32>(p, variant, args, grid, s, bpc, thr);
+    case SE2W_FP32_KAHAN: return ens_dispatch<MODE_F32K>(p, variant, args, grid, s, bpc, thr);
+    default: return -1;
+  }
+}
+
+int ens_launch(int p, int mode, int variant, const void *args, int grid, hipStream_t s) {
+  if (!args || grid < 8 || grid % 8) return -1;
+  int bpc = 0, thr = 0;
+  return ens_dispatch_mode(p, mode, variant, args, grid, s, &bpc, &thr);
+}
+
+int ens_occupancy(int p, int mode, int variant, int *blocks_per_cu, int *threads) {
+  return ens_dispatch_mode(p, mode, variant, nullptr, 0, nullptr, blocks_per_cu, threads);
 }
 }  // namespace se2w
 
@@ -892,16 +874,10 @@ int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
   if (!c->have_tables || !c->have_cells || !c->have_mass)
     return fail(SE2W_ERR_STATE, "set tables, cells and mass before se2w_run");
   HIPCHK(hipSetDevice(c->device));
-  const size_t rs = real_size(c->mode);
-  int ierr;
+  if (int ierr = ensure_run_buffers(c, nsteps)) return ierr;
   // source-time table for this run
   if (c->ns > 0) {
     const size_t n = (size_t)nsteps * c->ns;
-    if (n > c->psi_cap) {
-      dev_free(c->d_psi);
-      if ((ierr = dev_alloc(&c->d_psi, n * rs))) return ierr;
-      c->psi_cap = n;
-    }
     if (c->mode == SE2W_FP64) {
       HIPCHK(hipMemcpyAsync(c->d_psi, psi, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
@@ -910,23 +886,6 @@ int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
       for (size_t i = 0; i < n; i++) tmp[i] = (float)psi[i];
       HIPCHK(hipMemcpyAsync(c->d_psi, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
       HIPCHK(hipStreamSynchronize(c->stream));
-    }
-  }
-  // receiver sample buffer
-  if (c->nr > 0) {
-    const size_t n = (size_t)nsteps * c->nr * rec_stride(c->p);
-    if (n > c->rec_cap) {
-      dev_free(c->d_rec);
-      if ((ierr = dev_alloc(&c->d_rec, n * rs))) return ierr;
-      c->rec_cap = n;
-    }
-  }
-  if (c->ner > 0) {
-    const size_t n = (size_t)nsteps * c->ner * c->npe * 4;
-    if (n > c->erec_cap) {
-      dev_free(c->d_erec);
-      if ((ierr = dev_alloc(&c->d_erec, n * rs))) return ierr;
-      c->erec_cap = n;
     }
   }
   c->rec_steps = nsteps;
@@ -938,6 +897,32 @@ int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
 }  // namespace
 
 extern "C" {
+
+int se2w_reserve(se2w_ctx *c, int32_t nsteps) {
+  if (!c) return fail(SE2W_ERR_ARG, "null context");
+  if (nsteps < 1) return fail(SE2W_ERR_ARG, "nsteps must be >= 1");
+  HIPCHK(hipSetDevice(c->device));
+  if (int ierr = ensure_run_buffers(c, nsteps)) return ierr;
+  return ensure_stage_for(c, nsteps);
+}
+
+int se2w_get_counts(se2w_ctx *c, int32_t counts[4]) {
+  if (!c || !counts) return fail(SE2W_ERR_ARG, "null argument");
+  counts[0] = c->ns;
+  counts[1] = c->nr;
+  counts[2] = c->ner;
+  counts[3] = c->rec_steps;
+  return SE2W_OK;
+}
+
+int se2w_get_reserved(se2w_ctx *c, int64_t caps[4]) {
+  if (!c || !caps) return fail(SE2W_ERR_ARG, "null argument");
+  caps[0] = (int64_t)c->psi_cap;
+  caps[1] = (int64_t)c->rec_cap;
+  caps[2] = (int64_t)c->erec_cap;
+  caps[3] = (int64_t)c->stage_cap;
+  return SE2W_OK;
+}
 
 int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
   int ierr = prepare_run(c, nsteps, dt, psi);
@@ -984,6 +969,52 @@ int se2w_get_receivers(se2w_ctx *c, double *trace) {
   return SE2W_OK;
 }
 
+}  // extern "C"
+
+namespace {
+
+// The element kernel that f = K(u) runs on: the context's variant when that
+// variant has an element-only form (the LDS tile kernel, or the register
+// kernel's first schedule), else the LDS tile kernel.  The fused-only
+// variants (the other register schedules, the 512-thread tile, matrix-core
+// and line kernels) tile the mesh differently from the element-only kernel
+// they would fall back to, so the call switches the tiling with the kernel.
+int element_only_variant(const se2w_ctx *c) {
+  if (is_reg_variant(c->variant) && reg_available(c->p, c->mode) &&
+      reg_variant_built(c->p, c->variant, SE2W_SCHED_SPLIT))
+    return c->variant;
+  return SE2W_KERNEL_LDS;
+}
+
+// f = K(u) with u, f in the padded device layout of the context (Utmp -> F),
+// the launch tiled for the element-only kernel it runs.  The context's tiling
+// is restored before returning; the per-tile source and receiver lists are
+// not read by the element-only kernels.
+int enqueue_forces(se2w_ctx *c, void *u_dev, void *f_dev) {
+  const int v0 = c->variant;
+  c->variant = element_only_variant(c);
+  set_tiling(c);
+  if (c->mode == SE2W_FP64) {
+    StepParams<double> prm = make_params<MODE_F64>(c, 0, 1.0, 0);
+    prm.u_in = (const double2 *)u_dev;
+    prm.f = (double2 *)f_dev;
+    launch_element_only<MODE_F64>(launch_cfg(c), prm);
+  } else {
+    StepParams<float> prm = make_params<MODE_F32>(c, 0, 1.0, 0);
+    prm.u_in = (const float2 *)u_dev;
+    prm.f = (float2 *)f_dev;
+    launch_element_only<MODE_F32>(launch_cfg(c), prm);
+  }
+  c->variant = v0;
+  set_tiling(c);
+  HIPCHK(hipGetLastError());
+  return SE2W_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int se2w_assemble_forces(se2w_ctx *c, const double *u, double *f) {
   if (!c || !u || !f) return fail(SE2W_ERR_ARG, "null argument");
   if (!c->have_tables || !c->have_cells) return fail(SE2W_ERR_STATE, "set tables and cells first");
@@ -998,23 +1029,22 @@ int se2w_assemble_forces(se2w_ctx *c, const double *u, double *f) {
   if ((ierr = dev_alloc(&scratch.p, sizeof(double) * 2 * n))) return ierr;
   double *tmp = (double *)scratch.p;
   HIPCHK(hipMemcpyAsync(tmp, u, sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->stream));
-  if (c->mode == SE2W_FP64) {
+  // every owned node is written by the element kernel; the clear keeps a
+  // missed node from returning the previous call's value
+  HIPCHK(hipMemsetAsync(c->F, 0, field, c->stream));
+  if (c->mode == SE2W_FP64)
     hipLaunchKernelGGL(k_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (double2 *)c->Utmp,
                        c->nx, c->ny, c->pitch);
-    StepParams<double> prm = make_params<MODE_F64>(c, 0, 1.0, 0);
-    prm.u_in = (const double2 *)c->Utmp;
-    launch_element_only<MODE_F64>(c, prm);
-    hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)c->F,
-                       (const double2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
-  } else {
+  else
     hipLaunchKernelGGL(k_from_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (float2 *)c->Utmp, c->nx,
                        c->ny, c->pitch);
-    StepParams<float> prm = make_params<MODE_F32>(c, 0, 1.0, 0);
-    prm.u_in = (const float2 *)c->Utmp;
-    launch_element_only<MODE_F32>(c, prm);
+  if ((ierr = enqueue_forces(c, c->Utmp, c->F))) return ierr;
+  if (c->mode == SE2W_FP64)
+    hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)c->F,
+                       (const double2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
+  else
     hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)c->F,
                        (const float2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
-  }
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(f, tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1328,8 +1358,7 @@ int ens_run_mode(se2w_ensemble *e, int nsteps, const double *dt) {
   for (size_t i = 0; i < e->members.size(); i++) {
     se2w_ctx *c = e->members[i];
     StepParams<R> prm = make_params<MODE>(c, 0, dt[i], 0);
-    const long n = (long)c->pitch * c->ny;
-    hipLaunchKernelGGL((predictor_kernel<MODE>), dim3(grid_for(n)), dim3(kThreads), 0, e->stream, prm);
+    launch_predictor<MODE>(prm, (long)c->pitch * c->ny, e->stream);
     c->cur ^= 1;
   }
   HIPCHK(hipGetLastError());

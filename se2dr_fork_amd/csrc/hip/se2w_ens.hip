@@ -1,11 +1,16 @@
 // se2w_ens.hip -- batched (ensemble) step kernels: a persistent grid walks the
 // tiles of all members; each workgroup runs the single-instance tile bodies
-// with the step parameters of the member that owns the tile.
+// with the step parameters of the member that owns the tile.  Compiled once
+// per precision mode with -DSE2W_MODE_TU=<mode>.
 #include "se2w_ens.hpp"
 
 #include "se2w.h"
 #include "se2w_kernels_reg.hpp"
 #include "se2w_kernels_mfma.hpp"
+
+#ifndef SE2W_MODE_TU
+#error "compile with -DSE2W_MODE_TU=<mode>"
+#endif
 
 namespace se2w {
 
@@ -130,40 +135,22 @@ int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc
   return -1;
 }
 
-template <int P>
-int dispatch_p(int mode, int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
-  switch (mode) {
-    case SE2W_FP64: return dispatch_pm<P, MODE_F64>(variant, args, grid, s, bpc, thr);
-    case SE2W_FP32: return dispatch_pm<P, MODE_F32>(variant, args, grid, s, bpc, thr);
-    case SE2W_FP32_KAHAN: return dispatch_pm<P, MODE_F32K>(variant, args, grid, s, bpc, thr);
-    default: return -1;
-  }
-}
-
-int dispatch(int p, int mode, int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
-  switch (p) {
-    case 1: return dispatch_p<1>(mode, variant, args, grid, s, bpc, thr);
-    case 2: return dispatch_p<2>(mode, variant, args, grid, s, bpc, thr);
-    case 3: return dispatch_p<3>(mode, variant, args, grid, s, bpc, thr);
-    case 4: return dispatch_p<4>(mode, variant, args, grid, s, bpc, thr);
-    case 5: return dispatch_p<5>(mode, variant, args, grid, s, bpc, thr);
-    case 6: return dispatch_p<6>(mode, variant, args, grid, s, bpc, thr);
-    case 7: return dispatch_p<7>(mode, variant, args, grid, s, bpc, thr);
-    case 8: return dispatch_p<8>(mode, variant, args, grid, s, bpc, thr);
-    default: return -1;
-  }
-}
-
 }  // namespace
 
-int ens_launch(int p, int mode, int variant, const void *args, int grid, hipStream_t s) {
-  if (!args || grid < 8 || grid % 8) return -1;
-  int bpc = 0, thr = 0;
-  return dispatch(p, mode, variant, args, grid, s, &bpc, &thr);
-}
-
-int ens_occupancy(int p, int mode, int variant, int *blocks_per_cu, int *threads) {
-  return dispatch(p, mode, variant, nullptr, 0, nullptr, blocks_per_cu, threads);
+template <>
+int ens_dispatch<SE2W_MODE_TU>(int p, int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
+  constexpr int M = SE2W_MODE_TU;
+  switch (p) {
+    case 1: return dispatch_pm<1, M>(variant, args, grid, s, bpc, thr);
+    case 2: return dispatch_pm<2, M>(variant, args, grid, s, bpc, thr);
+    case 3: return dispatch_pm<3, M>(variant, args, grid, s, bpc, thr);
+    case 4: return dispatch_pm<4, M>(variant, args, grid, s, bpc, thr);
+    case 5: return dispatch_pm<5, M>(variant, args, grid, s, bpc, thr);
+    case 6: return dispatch_pm<6, M>(variant, args, grid, s, bpc, thr);
+    case 7: return dispatch_pm<7, M>(variant, args, grid, s, bpc, thr);
+    case 8: return dispatch_pm<8, M>(variant, args, grid, s, bpc, thr);
+    default: return -1;
+  }
 }
 
 }  // namespace se2w

@@ -32,4 +32,10 @@ int ens_launch(int p, int mode, int variant, const void *args, int grid, hipStre
 // workgroup size; -1 when the variant is not built for (p, mode).
 int ens_occupancy(int p, int mode, int variant, int *blocks_per_cu, int *threads);
 
+// The batched kernels of one precision mode (one translation unit each):
+// with args, launches the kernel of (p, variant); without, reports its
+// occupancy in *bpc and *thr.  Returns 0, or -1 when the variant is not built.
+template <int MODE>
+int ens_dispatch(int p, int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr);
+
 }  // namespace se2w

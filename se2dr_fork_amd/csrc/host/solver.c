@@ -166,6 +166,19 @@ int se2w_solver_get_uv_trace(SE2WSolver sv, double *trace)
   return 0;
 }
 
+int se2w_solver_reserve(SE2WSolver sv, int nsteps)
+{
+  const int nr = sv && sv->recv ? sv->recv->nr : 0;
+  int ierr;
+  if (!sv || nsteps < 1) return host_fail(SE2W_HOST_ERR_ARG, "bad reserve arguments");
+  if ((ierr = grow(&sv->times, &sv->times_cap, (size_t)nsteps))) return ierr;
+  if (sv->ns > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ns))) return ierr;
+  if (nr > 0 && (ierr = grow(&sv->trace, &sv->trace_cap, (size_t)nsteps * nr * 7))) return ierr;
+  if (sv->nuv > 0 && (ierr = grow(&sv->uv_trace, &sv->uv_cap, (size_t)nsteps * sv->nuv * 4))) return ierr;
+  if ((ierr = se2w_reserve(sv->ctx, nsteps))) return gpu_fail(ierr);
+  return 0;
+}
+
 int se2w_solver_advance(SE2WSolver sv, int nsteps, double dt)
 {
   const int nr = sv && sv->recv ? sv->recv->nr : 0;

@@ -15,8 +15,8 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from . import (FP64, SCHED_FUSED, SOURCE_FORCE, SOURCE_MOMENT, STF, Receivers, Solver, Source, SpecFE, _ptr,
-               libraries)
+from . import (FP64, IMPL_NEAREST_QPOINT, SCHED_FUSED, SOURCE_FORCE, SOURCE_MOMENT, STF, Receivers, Solver, Source,
+               SpecFE, _ptr, libraries)
 
 
 @dataclass
@@ -40,6 +40,7 @@ class Scenario:
     tmax: float = 1.0e30
     use_nearest_internal_gll: bool = True
     dt: Optional[float] = None              # explicit dt overrides cfl_factor * CFL
+    src_impl: Optional[List[int]] = None    # per source IMPL_NEAREST_QPOINT (default) or IMPL_POINTWISE
 
     @property
     def nnodes(self) -> int:
@@ -195,7 +196,8 @@ def build(sc: Scenario, precision=FP64, schedule=SCHED_FUSED, with_solver=True) 
     spec.set_cell_material_velocity(sc.vp, sc.vs, sc.rho)
     sources, stfs = [], []
     for s in range(len(sc.src_type)):
-        sources.append(Source(spec, sc.src_type[s], sc.src_xy[s], sc.src_values[s],
+        impl = IMPL_NEAREST_QPOINT if sc.src_impl is None else sc.src_impl[s]
+        sources.append(Source(spec, sc.src_type[s], sc.src_xy[s], sc.src_values[s], impl=impl,
                               use_nearest_internal_gll=sc.use_nearest_internal_gll))
         kind, a, b, c = sc.stf[s]
         stfs.append(STF(kind, a, b, c))

@@ -47,6 +47,11 @@ def small_cases():
         sc.use_nearest_internal_gll = p > 1
         yield sc
     yield SC.small(mx=33, my=17, p=4, nt=30, nsrc=3, nrec=9)  # several tiles, partial edge tiles
+    for kind, p in ((S.SOURCE_MOMENT, 4), (S.SOURCE_FORCE, 6)):  # SOURCE_IMPL_POINTWISE
+        sc = SC.small(mx=11, my=9, p=p, nt=30, nsrc=3, kind=kind, seed=50 + p)
+        sc.src_impl = [S.IMPL_POINTWISE] * 3
+        sc.name = "pointwise"
+        yield sc
 
 
 @pytest.mark.parametrize("sc", list(small_cases()), ids=lambda s: f"{s.name}-{s.mx}x{s.my}-p{s.p}")
@@ -156,6 +161,36 @@ def test_assemble_forces(oracle):
                 assert rel_l2(f, fo) < 2e-6
 
 
+@pytest.mark.parametrize("p, mx, my", [(4, 41, 37), (6, 23, 19), (8, 13, 11)])
+def test_assemble_forces_with_every_kernel_variant(oracle, p, mx, my):
+    """f = K(u) after set_kernel of each variant the context accepts: the call
+    runs an element-only kernel with that kernel's own tiling (fused-only
+    variants fall back to the LDS tile kernel), so every node of a
+    multi-tile mesh is assembled.  FP64 bit-identical to the oracle; FP32
+    within rounding."""
+    sc = SC.small(mx=mx, my=my, p=p, seed=90 + p)
+    u = np.random.default_rng(p).standard_normal(2 * sc.nnodes)
+    fo = oracle.Problem(sc, dt=1e-5).assemble(u)
+    hip, _ = S.libraries()
+    for prec in (S.FP64, S.FP32, S.FP32_KAHAN):
+        built = SC.build(sc, precision=prec)
+        ctx = built.solver.ctx
+        tried = 0
+        for k in range(hip.se2w_kernel_count()):
+            try:
+                ctx.set_kernel(k)
+            except S.Se2wError:
+                continue
+            tried += 1
+            f = ctx.assemble_forces(u)
+            if prec == S.FP64:
+                np.testing.assert_array_equal(f, fo)
+            else:
+                assert rel_l2(f, fo) < 2e-6, (prec, k)
+        assert tried >= (2 if prec == S.FP64 else 4)
+        built.solver.close()
+
+
 @pytest.mark.parametrize("p, mx, my", [(4, 37, 33), (6, 23, 19), (4, 3, 2), (4, 40, 1), (2, 9, 7), (3, 8, 5),
                                         (5, 11, 6), (7, 12, 9), (8, 17, 10)])
 def test_fp32_kernel_variants_agree(p, mx, my):
@@ -196,6 +231,27 @@ def test_autotune_keeps_the_state():
     _, r2 = b2.solver.advance(30, b2.dt)
     np.testing.assert_array_equal(r1, r2)
     np.testing.assert_array_equal(u1, b2.solver.ctx.get_state()[0])
+
+
+def test_reserve_makes_advances_allocation_free():
+    """After reserve(n), advances of up to n steps do not grow any run
+    buffer, and their results equal those of an unreserved solver."""
+    sc = SC.small(mx=9, my=7, p=4, nt=90)
+    rows = []
+    for reserve in (True, False):
+        b = SC.build(sc, precision=S.FP32_KAHAN)
+        b.solver.set_uv_receivers([S.ReceiverUV(b.spec, [1500.3, 2200.7], interpolate=True)])
+        if reserve:
+            b.solver.reserve(40)
+            caps = b.solver.ctx.reserved()
+            ns, nr, ner, _ = b.solver.ctx.counts()
+            assert caps[0] >= 40 * ns and caps[1] >= 40 * nr * (6 + 2 * 5) and caps[2] >= 40 * ner * 25 * 4
+        out = [b.solver.advance(n, b.dt)[1] for n in (40, 5, 40)]
+        if reserve:
+            assert b.solver.ctx.reserved() == caps
+        rows.append(np.concatenate(out))
+        b.solver.close()
+    np.testing.assert_array_equal(rows[0], rows[1])
 
 
 def test_norms():
@@ -292,6 +348,48 @@ def test_config2_full_size(oracle):
     assert rel_l2(r32k[..., 2:4], r64[..., 2:4]) < 1e-5
 
 
+def _config5_with_signal():
+    """The 2048x2048, p=8 layered workload (268M nodes) with its four Ricker
+    sources peaking at t = 0 and one receiver 0.7-3.5 m from each source, so
+    that 500 steps (0.7 ms simulated) carry signal to every receiver."""
+    sc = SC.config5()
+    sc.stf = [("ricker", 0.0, f, a) for (_, _, f, a) in sc.stf]
+    off = np.array([[0.7, 0.0], [0.0, 1.9], [-2.6, 1.1], [2.4, -2.5]])
+    sc.rec_xy = sc.src_xy + off
+    return sc
+
+
+@pytest.mark.timeout(900)
+def test_config5_full_size(oracle):
+    """2048x2048 elements, p=8 (268,468,225 nodes, FP64 state 4.3 GB per
+    field).  FP64: two steps bit-identical to the oracle (state and receiver
+    rows).  FP32 and FP32+Kahan: the whole u, v state and the receiver
+    displacement/velocity against FP64 after the workload's 500 steps."""
+    sc = _config5_with_signal()
+    b, _, r2, u, v, a = gpu_run(sc, nsteps=2)
+    print(f"config5: GPU FP64 2 steps, dt = {b.dt:.6e}", flush=True)
+    pr = oracle.Problem(sc, dt=b.dt, nt=2)
+    _, ro, uo, vo, ao = pr.run()
+    print("config5: oracle 2 steps", flush=True)
+    assert np.abs(uo).max() > 0.0 and np.abs(ro[..., :4]).max() > 0.0
+    np.testing.assert_array_equal(r2, ro)
+    for x, y in ((u, uo), (v, vo), (a, ao)):
+        assert np.array_equal(x, y)
+    del u, v, a, uo, vo, ao, pr
+    b.solver.close()
+    b64, _, r64, u64, v64, _ = gpu_run(sc, nsteps=500)
+    b64.solver.close()
+    print("config5: GPU FP64 500 steps", flush=True)
+    for prec, tol in ((S.FP32, 2e-5), (S.FP32_KAHAN, 2e-5)):
+        b32, _, r32, u32, v32, _ = gpu_run(sc, precision=prec, nsteps=500)
+        b32.solver.close()
+        eu, ev = rel_l2(u32, u64), rel_l2(v32, v64)
+        er = rel_l2(r32[..., :4], r64[..., :4])
+        print(f"config5: precision {prec}: state u {eu:.2e} v {ev:.2e}, receivers {er:.2e}", flush=True)
+        assert eu < tol and ev < tol and er < tol
+        del u32, v32
+
+
 def test_gpu_error_paths():
     ctx = S.Context(4, 4, 3, precision=S.FP32)
     with pytest.raises(S.Se2wError, match="set tables, cells and mass"):
@@ -331,36 +429,30 @@ def test_mfma_kernel_is_rejected_where_not_built():
         b64.solver.ctx.set_kernel(S.KERNEL_MFMA)
 
 
-@pytest.mark.parametrize("prec", [S.FP64, S.FP32])
-def test_single_station_receivers_sample_the_state(prec):
-    """RecordUV / RecordUV_interp on the GPU: each step's row is the weighted
-    sum, in the reference's node order, of the u and v that step leaves."""
-    sc = SC.small(mx=9, my=7, p=5, nt=10, seed=44)
-    b = SC.build(sc, precision=prec)
-    stations = [S.ReceiverUV(b.spec, [1500.3, 2200.7], interpolate=False),
-                S.ReceiverUV(b.spec, [1500.3, 2200.7], interpolate=True),
-                S.ReceiverUV(b.spec, [3110.0, 640.2], interpolate=True)]
-    b.solver.set_uv_receivers(stations)
-    el = b.spec.element_map()
-    for step in range(4):
-        b.solver.advance(1, b.dt)
-        rows = b.solver.uv_trace(1)[0]
-        u, v, _ = b.solver.ctx.get_state()
-        for r, st in enumerate(stations):
-            e, w, _ = st.get()
-            want = [0.0, 0.0, 0.0, 0.0]
-            for q, n in enumerate(el[e]):
-                vals = (u[2 * n], u[2 * n + 1], v[2 * n], v[2 * n + 1])
-                for f in range(4):
-                    want[f] = want[f] + float(w[q]) * float(vals[f])
-            if prec == S.FP64:
-                assert list(rows[r]) == want
-            else:
-                np.testing.assert_allclose(rows[r], want, rtol=1e-6, atol=1e-6 * np.abs(want).max() + 1e-300)
-    # the nearest-node station returns that node's values exactly
-    e, w, _ = stations[0].get()
-    n = el[e][int(np.argmax(w))]
-    assert list(b.solver.uv_trace(1)[0][0]) == [u[2 * n], u[2 * n + 1], v[2 * n], v[2 * n + 1]]
+@pytest.mark.parametrize("p", [3, 5])
+def test_single_station_receivers_match_the_oracle(oracle, p):
+    """RecordUV / RecordUV_interp on the GPU against the oracle's restatement
+    (se2wave.c:2351-2581): FP64 rows bit-identical at every step, over runs
+    advanced in chunks; FP32+Kahan rows within its displacement/velocity
+    tolerance of the FP64 rows."""
+    sc = SC.small(mx=9, my=7, p=p, nt=24, seed=44 + p)
+    xy = [[1500.3, 2200.7], [1500.3, 2200.7], [3110.0, 640.2], [0.0, 3999.0], [2222.2, 1777.7]]
+    interp = [0, 1, 1, 0, 1]
+    rows = {}
+    for prec in (S.FP64, S.FP32_KAHAN):
+        b = SC.build(sc, precision=prec)
+        b.solver.set_uv_receivers([S.ReceiverUV(b.spec, xr, interpolate=bool(i)) for xr, i in zip(xy, interp)])
+        parts = []
+        for n in (1, 7, 16):
+            b.solver.advance(n, b.dt)
+            parts.append(b.solver.uv_trace(n))
+        rows[prec] = np.concatenate(parts)
+        b.solver.close()
+    pr = oracle.Problem(sc, dt=b.dt, nt=24, uv_xy=xy, uv_interp=interp)
+    _, _, _, _, _, uvo = pr.run(want_uv=True)
+    assert rows[S.FP64].shape == uvo.shape == (24, 5, 4)
+    np.testing.assert_array_equal(rows[S.FP64], uvo)
+    assert rel_l2(rows[S.FP32_KAHAN], uvo) < 1e-5
 
 
 @pytest.mark.parametrize("p", [3, 8])

@@ -23,6 +23,11 @@ def scenarios():
     sc = SC.small(mx=4, my=4, p=1)
     sc.use_nearest_internal_gll = False
     yield sc
+    for kind, p in ((S.SOURCE_MOMENT, 5), (S.SOURCE_FORCE, 3), (S.SOURCE_MOMENT, 1)):
+        sc = SC.small(mx=6, my=5, p=p, kind=kind, nsrc=3, seed=30 + p)
+        sc.src_impl = [S.IMPL_POINTWISE] * 3  # basis at the source's own local coordinate
+        sc.name = "pointwise"
+        yield sc
 
 
 @pytest.mark.parametrize("sc", list(scenarios()), ids=lambda s: f"{s.name}-{s.mx}x{s.my}-p{s.p}")
@@ -180,6 +185,22 @@ def test_perturbed_material_follows_the_reference_draws():
     np.testing.assert_array_equal(r, rho)
     np.testing.assert_array_equal(mu, vs * vs * rho)
     np.testing.assert_array_equal(lam, vp * vp * rho - 2.0 * (vs * vs * rho))
+
+
+@pytest.mark.parametrize("p", [2, 5, 8])
+def test_single_station_weights_bit_identical_to_oracle(oracle, p):
+    """The product's RecordUV / RecordUV_interp stations (element and nodal
+    weights handed to the GPU) equal the oracle's restatement bit for bit."""
+    sc = SC.small(mx=7, my=6, p=p, seed=40 + p)
+    b = SC.build(sc, with_solver=False)
+    xy = [[1500.3, 2200.7], [3110.0, 640.2], [0.0, 0.0], [3999.0, 3999.5], [571.42857142857144, 1333.3333333333333]]
+    pr = oracle.Problem(sc, dt=1e-5, uv_xy=xy + xy, uv_interp=[0] * len(xy) + [1] * len(xy))
+    _, el, w = pr.uv_rows()
+    for r, xr in enumerate(xy + xy):
+        st = S.ReceiverUV(b.spec, xr, interpolate=r >= len(xy))
+        e, wp, _ = st.get()
+        assert e == el[r]
+        np.testing.assert_array_equal(wp, w[r])
 
 
 def test_single_station_receivers(tmp_path):

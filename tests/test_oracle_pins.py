@@ -261,6 +261,133 @@ def test_receivers_brute_force(oracle):
         assert nid[r] == cand[int(np.argmin(d2))]
 
 
+def _element_nodes(sc, e):
+    """Global node indices of element e in element-local order k = i + j (p+1)."""
+    nb, nx = sc.p + 1, sc.p * sc.mx + 1
+    ei, ej = e % sc.mx, e // sc.mx
+    return np.array([(ei * sc.p + k % nb) + (ej * sc.p + k // nb) * nx for k in range(nb * nb)])
+
+
+@pytest.mark.parametrize("p", [2, 3, 5, 8])
+def test_receiver_rows_and_curl_closed_forms(oracle, p):
+    """Receiver rows (se2wave.c:2908-2959) for velocity fields of known curl
+    dvy/dx - dvx/dy: v = w (-y, x) + b (x, y) + c gives 2w, v = (0, g x) gives
+    g and v = (g y, 0) gives -g.  A flipped sign or a swapped derivative fails
+    one of the three; the first six columns are the nearest node's u, v, a."""
+    sc = _scenario(mx=6, my=5, p=p, seed=8, nrec=7)
+    pr = oracle.Problem(sc, dt=1e-5)
+    c, _, _ = pr.coords()
+    nid, _, _ = pr.receivers()
+    rng = np.random.default_rng(p)
+    u, a = rng.standard_normal(c.size), rng.standard_normal(c.size)
+    x, y = c[0::2], c[1::2]
+    cases = []
+    w, b, g = 0.37, 0.25, -1.9
+    v = np.zeros_like(c)
+    v[0::2], v[1::2] = -w * y + b * x + 3.0, w * x + b * y - 2.0
+    cases.append((v, 2 * w))
+    v = np.zeros_like(c)
+    v[1::2] = g * x
+    cases.append((v, g))
+    v = np.zeros_like(c)
+    v[0::2] = g * y
+    cases.append((v, -g))
+    for v, curl in cases:
+        rows = pr.receiver_rows(u, v, a)
+        for r, n in enumerate(nid):
+            assert list(rows[r, :6]) == [u[2 * n], u[2 * n + 1], v[2 * n], v[2 * n + 1], a[2 * n], a[2 * n + 1]]
+        np.testing.assert_allclose(rows[:, 6], curl, rtol=1e-10)
+
+
+@pytest.mark.parametrize("p", [2, 4, 5])
+def test_single_station_rows(oracle, p):
+    """RecordUV (se2wave.c:2351-2443): the nearest node of the containing
+    element, by brute force; RecordUV_interp (2445-2581): Lagrange
+    interpolation, which reproduces a field of degree <= p in each coordinate
+    at the station exactly (the closed form fails a wrong local coordinate,
+    a swapped xi/eta or a wrong node order)."""
+    sc = _scenario(mx=6, my=5, p=p, seed=9)
+    xy = [[1500.3, 2200.7], [3110.0, 640.2], [0.0, 0.0], [3999.0, 3999.5], [2000.0, 800.0]]
+    n = len(xy)
+    pr = oracle.Problem(sc, dt=1e-5, uv_xy=xy + xy, uv_interp=[0] * n + [1] * n)
+    c, _, gmax = pr.coords()
+    x, y = c[0::2] / 4000.0, c[1::2] / 4000.0
+
+    def field(X, Y):
+        return np.array([X**p + 0.5 * Y - 0.25, X * Y ** (p - 1) + 2.0, (X - 0.5) ** 2 * Y, -(Y**p) * X + X])
+
+    F = field(x, y)
+    u, v = np.zeros_like(c), np.zeros_like(c)
+    u[0::2], u[1::2], v[0::2], v[1::2] = F
+    rows, el, w = pr.uv_rows(u, v)
+    for r, (X, Y) in enumerate(xy):
+        ei, ej = int(X / (gmax[0] / sc.mx)), int(Y / (gmax[1] / sc.my))
+        assert el[r] == el[r + n] == ei + ej * sc.mx
+        nodes = _element_nodes(sc, el[r])
+        d2 = (X - c[2 * nodes]) ** 2 + (Y - c[2 * nodes + 1]) ** 2
+        k = int(np.argmin(d2))
+        m = nodes[k]
+        assert list(rows[r]) == [u[2 * m], u[2 * m + 1], v[2 * m], v[2 * m + 1]]
+        assert w[r, k] == 1.0 and w[r].sum() == 1.0
+        np.testing.assert_allclose(rows[r + n], field(X / 4000.0, Y / 4000.0), rtol=1e-11, atol=1e-12)
+        assert abs(w[r + n].sum() - 1.0) < 1e-12
+
+
+@pytest.mark.parametrize("p", [2, 4, 7])
+def test_pointwise_sources_closed_forms(oracle, p):
+    """SOURCE_IMPL_POINTWISE (EXT of se2wave.c:3210-3211, set up as
+    3339-3342 + 3367-3429 at the source's own local coordinate 3097-3104).
+    For a field u of degree <= p in each coordinate the moment source's nodal
+    values give sum_k vals_k . u_k = M : grad u at the source point and the
+    point force's give F . u at the source point.  The source point enters
+    both, so a wrong local coordinate, a transposed moment tensor or a missing
+    Jacobian fails them."""
+    import se2dr_fork_amd as S
+
+    for kind in (S.SOURCE_MOMENT, S.SOURCE_FORCE):
+        sc = _scenario(mx=5, my=4, p=p, seed=12, nsrc=3, kind=kind)
+        sc.src_values = np.array([[1.3, -0.4, 0.7, 2.1], [-0.2, 0.9, 1.1, 0.3], [0.5, 0.5, -1.5, 0.8]])
+        if kind == S.SOURCE_FORCE:
+            sc.src_values[:, 2:] = 0.0
+        sc.src_impl = [S.IMPL_POINTWISE] * 3
+        pr = oracle.Problem(sc, dt=1e-5)
+        c, _, _ = pr.coords()
+        L = 4000.0
+        x, y = c[0::2] / L, c[1::2] / L
+        ux, uy = x**2 * y + 0.3 * y**p, x**p - 2.0 * x * y
+        for s in range(3):
+            e, g, vals = pr.source(s)
+            assert g == -1
+            nodes = _element_nodes(sc, e)
+            lhs = np.dot(vals[0::2], ux[nodes]) + np.dot(vals[1::2], uy[nodes])
+            X, Y = sc.src_xy[s] / L
+            if kind == S.SOURCE_MOMENT:
+                Mxx, Mxy, Myx, Myy = sc.src_values[s]
+                dux = (2 * X * Y / L, (X**2 + 0.3 * p * Y ** (p - 1)) / L)
+                duy = (((p * X ** (p - 1)) - 2.0 * Y) / L, -2.0 * X / L)
+                rhs = Mxx * dux[0] + Mxy * dux[1] + Myx * duy[0] + Myy * duy[1]
+            else:
+                rhs = sc.src_values[s][0] * (X**2 * Y + 0.3 * Y**p) + sc.src_values[s][1] * (X**p - 2.0 * X * Y)
+            assert lhs == pytest.approx(rhs, rel=1e-10, abs=1e-12 * np.abs(vals).max())
+
+
+def test_pointwise_source_on_a_node_equals_the_nearest_node_source(oracle):
+    """A pointwise source placed on an interior GLL node of its element has
+    the nodal values of the nearest-node source there (the two local
+    coordinates differ only by rounding)."""
+    import se2dr_fork_amd as S
+
+    sc = _scenario(mx=5, my=4, p=4, seed=13, nsrc=1)
+    c, _, _ = oracle.Problem(sc, dt=1e-5).coords()
+    node = _element_nodes(sc, 7)[2 + 1 * 5]
+    sc.src_xy = np.array([[c[2 * node], c[2 * node + 1]]])
+    near = oracle.Problem(sc, dt=1e-5).source(0)
+    sc.src_impl = [S.IMPL_POINTWISE]
+    point = oracle.Problem(sc, dt=1e-5).source(0)
+    assert near[0] == point[0] == 7 and near[1] == 2 + 1 * 5 and point[1] == -1
+    np.testing.assert_allclose(point[2], near[2], rtol=1e-10, atol=1e-12 * np.abs(near[2]).max())
+
+
 # --------------------------------------------------------------------------
 # Whole time loop
 # --------------------------------------------------------------------------
