@@ -112,6 +112,10 @@ def measured_traffic(mx, p, prec, signature):
 # library's variant order (SE2W_KERNEL_REG + k)
 REG_SCHEDULES = [(32, 4, 0, 2, 1), (16, 4, 0, 2, 1), (32, 2, 0, 2, 1), (32, 4, 0, 0, 2), (16, 4, 0, 4, 2),
                  (16, 8, 0, 4, 2), (16, 4, 0, 2, 2), (16, 4, 0, 0, 2)]
+# (elements in x, elements in y, minimum waves per SIMD, node-load placement)
+# of the FP64 register kernel's schedules (SE2W_KERNEL_REG64 + k)
+REG64_SCHEDULES = [(32, 4, 3, 0), (32, 4, 0, 0), (16, 8, 0, 4), (32, 4, 3, 4)]
+KERNEL_REG64 = 12
 
 
 def kernel_signature(variant, p, mode):
@@ -124,6 +128,9 @@ def kernel_signature(variant, p, mode):
         return f"line_kernel<{p}, {mode}>"
     if variant == 11:
         return f"tile_kernel<{p}, {mode}, true, 512>"
+    if variant >= KERNEL_REG64:
+        ex, ey, w, pf = REG64_SCHEDULES[variant - KERNEL_REG64]
+        return f"reg64_kernel<{p}, {ex}, {ey}, {w}, {pf}>"
     ex, ey, w, pf, lpe = REG_SCHEDULES[variant - 1]
     return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}, {pf}, {lpe}>"
 
@@ -145,6 +152,8 @@ def ens_kernel_signature(variant, p, mode):
 def kernel_name(variant, p, mode):
     what = ("LDS-staged tile" if variant == 0 else "matrix-core contractions" if variant == 9 else
             "line contractions" if variant == 10 else "LDS-staged tile, 512 threads" if variant == 11 else
+            "FP64 lane pair per element in the reference's operation order, %dx%d elements" %
+            REG64_SCHEDULES[variant - KERNEL_REG64][:2] if variant >= KERNEL_REG64 else
             "%s per element, %dx%d elements" % (("thread", "lane pair")[REG_SCHEDULES[variant - 1][4] - 1],
                                                 *REG_SCHEDULES[variant - 1][:2]))
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
@@ -375,17 +384,21 @@ def beyond_cache_bench(S, SC, mx=1024, p=4, precs=(0, 2, 1), steps=100, warmup=1
     return out
 
 
-def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
+def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200, f0=None, timing=True, n_fp64=8):
     """The learned surrogate: training data from the batched solver (96^2,
     p=4, FP32), the MLP trained on the GPU, its held-out error against the
-    solver and its time per answer against the solver's time per solve."""
+    solver and its time per answer against the solver's time per solve.  The
+    batched FP32 solves that make the data are checked against FP64 solves
+    of n_fp64 of the held-out instances.  f0: the Ricker peak frequency of
+    the sources (default: the surrogate's own, 1 Hz)."""
     from se2dr_fork_amd import surrogate as SG
 
-    log("surrogate: generating the training set")
-    tr = SG.generate(range(ntrain), log=log)
-    te = SG.generate(range(100000, 100000 + ntest), log=log)
+    f0 = SG.F0 if f0 is None else f0
+    log(f"surrogate ({f0:g} Hz): generating the training set")
+    tr = SG.generate(range(ntrain), log=log, f0=f0)
+    te = SG.generate(range(100000, 100000 + ntest), log=log, f0=f0)
     log("surrogate: training")
-    sg = SG.Surrogate()
+    sg = SG.Surrogate(f0=f0)
     losses = sg.fit(tr, epochs=epochs, lr=1e-3)
     train_ms = sg.mlp.timing()[0]
     sg.predict(te.x)  # warm-up
@@ -395,21 +408,30 @@ def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
     fwd_ms = sg.mlp.timing()[1]
     sg.close()
     # the plain solver one instance at a time (what the surrogate replaces)
-    nseq = 8
+    nseq = 8 if timing else 0
     t_seq = 0.0
     for k in range(nseq):
-        sc, _ = SG.instance(100000 + k)
+        sc, _ = SG.instance(100000 + k, f0=f0)
         b = SC.build(sc, precision=S.FP32)
+        b.solver.reserve(SG.STRIDE * SG.NSAMP)
         t0 = time.perf_counter()
         b.solver.advance(SG.STRIDE * SG.NSAMP, b.dt)
         t_seq += time.perf_counter() - t0
         b.solver.close()
-    per_solve_seq = t_seq / nseq
+    # the batched FP32 solves that made the data against FP64 solves
+    e64 = []
+    for k in range(n_fp64):
+        sc, _ = SG.instance(100000 + k, f0=f0)
+        b = SC.build(sc, precision=S.FP64)
+        _, tr64 = b.solver.advance(SG.STRIDE * SG.NSAMP, b.dt)
+        b.solver.close()
+        e64.append(SG.rel_l2(te.y[k], SG.seismograms(tr64)))
     per_solve_batched = te.seconds / ntest
     per_answer = t_wall / ntest
     d = ((te.x[:, None, :2] - tr.x[None, :, :2]) ** 2).sum(-1)
-    return {"workload": f"{ntrain} training + {ntest} held-out FP32 solves, 96x96 elements, p=4, 1 Hz Ricker point "
+    out = {"workload": f"{ntrain} training + {ntest} held-out FP32 solves, 96x96 elements, p=4, {f0:g} Hz Ricker point "
                         f"force, Vp = 5000 (1 + d), d ~ N(0, 5%) on 8x8 blocks; 4 receivers x (vx, vy) x 512 samples",
+           "f0_hz": f0,
             "network": "66-1024-1024-1024-4096 GELU, BF16 MFMA / FP32 accumulate, Adam lr 1e-3 cosine-decayed, batch 256, "
                        f"{epochs} epochs, mirror-symmetry augmentation (4x)",
             "held_out_rel_l2": SG.rel_l2(yp, te.y),
@@ -419,9 +441,15 @@ def surrogate_bench(S, SC, ntrain=4096, ntest=512, epochs=200):
             "final_training_loss": float(losses[-1]),
             "data_generation_s": tr.seconds + te.seconds, "training_s": train_ms / 1e3,
             "surrogate_s_per_answer_wall": per_answer, "surrogate_device_ms_for_held_out": fwd_ms,
-            "solver_s_per_solve_sequential": per_solve_seq, "solver_s_per_solve_batched": per_solve_batched,
-            "speedup_vs_sequential_solver": per_solve_seq / per_answer,
-            "speedup_vs_batched_solver": per_solve_batched / per_answer}
+            "solver_s_per_solve_batched": per_solve_batched,
+            "speedup_vs_batched_solver": per_solve_batched / per_answer,
+            "training_solves_rel_l2_vs_fp64": {"solves": n_fp64, "max": max(e64) if e64 else None,
+                                               "median": float(np.median(e64)) if e64 else None}}
+    if nseq:
+        per_solve_seq = t_seq / nseq
+        out["solver_s_per_solve_sequential"] = per_solve_seq
+        out["speedup_vs_sequential_solver"] = per_solve_seq / per_answer
+    return out
 
 
 def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0):
@@ -510,12 +538,17 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--ensemble-members", type=int, default=256)
     ap.add_argument("--no-surrogate", action="store_true", help="skip the surrogate measurement")
+    ap.add_argument("--surrogate-frequencies", default="",
+                    help="only the surrogate at each of these Ricker frequencies (Hz, comma-separated): held-out "
+                         "error of a network trained on that frequency's solves (one JSON line)")
     ap.add_argument("--variants", nargs=3, type=int, metavar=("MX", "P", "STEPS"),
                     help="only time every element-kernel variant at MX x MX elements of degree P for STEPS steps "
                          "(2048 8 STEPS: the layered 2048^2 p=8 workload) and print that as JSON")
     ap.add_argument("--accuracy", default="",
                     help="comma-separated step counts: receiver error of FP32 and FP32+Kahan against FP64 "
                          "on the benchmark workload after each")
+    ap.add_argument("--variant-precisions", default="fp32,fp32k",
+                    help="with --variants: the precision modes to time (fp64, fp32, fp32k)")
     ap.add_argument("--variants-check", type=int, default=0,
                     help="with --variants: receiver error of each variant after this many steps against FP64")
     ap.add_argument("--ensemble-steps", type=int, default=1500)
@@ -536,7 +569,12 @@ def main():
 
     if args.variants:
         mx, p, steps = args.variants
-        print(json.dumps(kernel_variant_bench(S, SC, mx, p, steps, ("fp32", "fp32k"), args.variants_check)))
+        print(json.dumps(kernel_variant_bench(S, SC, mx, p, steps, tuple(args.variant_precisions.split(",")),
+                                              args.variants_check)))
+        return
+    if args.surrogate_frequencies:
+        rows = [surrogate_bench(S, SC, f0=float(f), timing=False) for f in args.surrogate_frequencies.split(",")]
+        print(json.dumps({"surrogate_by_frequency": rows}))
         return
     if args.accuracy:
         print(json.dumps(accuracy_study(S, SC, [int(x) for x in args.accuracy.split(",")], mx=args.mx, p=args.p)))

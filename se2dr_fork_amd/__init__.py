@@ -21,7 +21,7 @@ LIB_DIR = os.path.join(_HERE, "lib")
 BIN_DIR = os.path.join(_HERE, "bin")
 
 FP64, FP32, FP32_KAHAN = 0, 1, 2
-KERNEL_LDS, KERNEL_REG, KERNEL_MFMA, KERNEL_LINE, KERNEL_LDS_WIDE = 0, 1, 9, 10, 11
+KERNEL_LDS, KERNEL_REG, KERNEL_MFMA, KERNEL_LINE, KERNEL_LDS_WIDE, KERNEL_REG64 = 0, 1, 9, 10, 11, 12
 SCHED_FUSED, SCHED_SPLIT = 0, 1
 SOURCE_FORCE, SOURCE_MOMENT = 0, 1
 IMPL_POINTWISE, IMPL_NEAREST_QPOINT, IMPL_DEFAULT = 0, 1, 3

@@ -16,6 +16,7 @@
 #include "se2w_kernels.hpp"
 #include "se2w_kernels_reg.hpp"
 #include "se2w_kernels_mfma.hpp"
+#include "se2w_kernels_reg64.hpp"
 #include "se2w_ens.hpp"
 #include "se2w_launch.hpp"
 
@@ -227,6 +228,16 @@ bool mfma_available(int p, int mode, int schedule) {
 int reg_schedule_of(int variant) { return variant - SE2W_KERNEL_REG; }
 bool is_reg_variant(int variant) { return variant >= SE2W_KERNEL_REG && variant < SE2W_KERNEL_REG + kNumRegSchedules; }
 
+// variant = SE2W_KERNEL_REG64 + schedule index into kReg64Schedules (FP64)
+static_assert(SE2W_KERNEL_REG64 == SE2W_KERNEL_LDS_WIDE + 1, "variant numbering");
+bool is_reg64_variant(int variant) {
+  return variant >= SE2W_KERNEL_REG64 && variant < SE2W_KERNEL_REG64 + kNumReg64Schedules;
+}
+bool reg64_variant_built(int p, int mode, int variant, int schedule) {
+  return mode == SE2W_FP64 && is_reg64_variant(variant) &&
+         reg64_schedule_built(p, variant - SE2W_KERNEL_REG64, schedule == SE2W_SCHED_FUSED);
+}
+
 void reg_shape(int variant, int &tex, int &tey) {
   const RegSchedule &r = kRegSchedules[reg_schedule_of(variant)];
   tex = r.ex - 1;
@@ -283,6 +294,10 @@ void build_tile_lists(const se2w_ctx *c, const std::vector<int> &elem, std::vect
 
 void set_tiling(se2w_ctx *c) {
   if (is_reg_variant(c->variant)) reg_shape(c->variant, c->tex, c->tey);
+  else if (is_reg64_variant(c->variant)) {
+    c->tex = kReg64Schedules[c->variant - SE2W_KERNEL_REG64].ex - 1;
+    c->tey = kReg64Schedules[c->variant - SE2W_KERNEL_REG64].ey - 1;
+  }
   else if (c->variant == SE2W_KERNEL_MFMA || c->variant == SE2W_KERNEL_LINE) mfma_shape(c->p, c->tex, c->tey);
   else tile_shape(c->p, c->mode, c->tex, c->tey);
   c->tiles_x = (c->mx + c->tex - 1) / c->tex;
@@ -528,7 +543,8 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
   c->device = desc->device;
   c->variant = reg_available(c->p, c->mode) && reg_variant_built(c->p, SE2W_KERNEL_REG, c->schedule)
                    ? SE2W_KERNEL_REG
-                   : SE2W_KERNEL_LDS;
+               : reg64_variant_built(c->p, c->mode, SE2W_KERNEL_REG64, c->schedule) ? SE2W_KERNEL_REG64
+                                                                                    : SE2W_KERNEL_LDS;
   set_tiling(c);
   int ierr = SE2W_OK;
   hipError_t he = hipSetDevice(c->device);
@@ -1095,8 +1111,11 @@ int se2w_get_timing(se2w_ctx *c, double *run_ms, double kernel_ms[2]) {
 int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   if (variant != SE2W_KERNEL_LDS && !is_reg_variant(variant) && variant != SE2W_KERNEL_MFMA &&
-      variant != SE2W_KERNEL_LINE && variant != SE2W_KERNEL_LDS_WIDE)
+      variant != SE2W_KERNEL_LINE && variant != SE2W_KERNEL_LDS_WIDE && !is_reg64_variant(variant))
     return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
+  if (is_reg64_variant(variant) && !reg64_variant_built(c->p, c->mode, variant, c->schedule))
+    return fail(SE2W_ERR_UNSUPPORTED, "the FP64 register-blocked kernel needs FP64, 2 <= p <= %d and the fused schedule",
+                kReg64MaxP);
   if (variant == SE2W_KERNEL_LDS_WIDE && c->schedule != SE2W_SCHED_FUSED)
     return fail(SE2W_ERR_UNSUPPORTED, "the 512-thread tile kernel is built for the fused schedule");
   if (variant == SE2W_KERNEL_MFMA && !mfma_available(c->p, c->mode, c->schedule))
@@ -1128,7 +1147,7 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   return SE2W_OK;
 }
 
-int se2w_kernel_count(void) { return SE2W_KERNEL_LDS_WIDE + 1; }
+int se2w_kernel_count(void) { return SE2W_KERNEL_REG64 + kNumReg64Schedules; }
 
 // Autotuner: times every variant built for the context's (p, precision,
 // schedule) on the context's own mesh and keeps the fastest.  The state is
@@ -1210,6 +1229,8 @@ int se2w_autotune(se2w_ctx *c, int32_t steps, int32_t reps, int32_t *best_varian
   if (reg_available(c->p, c->mode))
     for (int k = 0; k < kNumRegSchedules; k++)
       if (reg_variant_built(c->p, SE2W_KERNEL_REG + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG + k);
+  for (int k = 0; k < kNumReg64Schedules; k++)
+    if (reg64_variant_built(c->p, c->mode, SE2W_KERNEL_REG64 + k, c->schedule)) cands.push_back(SE2W_KERNEL_REG64 + k);
   if (mfma_available(c->p, c->mode, c->schedule)) {
     cands.push_back(SE2W_KERNEL_MFMA);
     cands.push_back(SE2W_KERNEL_LINE);

@@ -6,6 +6,7 @@
 #include "se2w.h"
 #include "se2w_kernels_mfma.hpp"
 #include "se2w_kernels_reg.hpp"
+#include "se2w_kernels_reg64.hpp"
 
 #ifndef SE2W_MODE_TU
 #error "compile with -DSE2W_MODE_TU=<mode>"
@@ -18,6 +19,22 @@ constexpr int kMode = SE2W_MODE_TU;
 using R = typename RealOf<kMode>::T;
 
 [[maybe_unused]] bool is_reg(int variant) { return variant >= SE2W_KERNEL_REG && variant < SE2W_KERNEL_REG + kNumRegSchedules; }
+
+template <int P, int S>
+bool launch_reg64_s(const LaunchCfg &c, const StepParams<R> &prm) {
+  if constexpr (kMode == MODE_F64 && S < kNumReg64Schedules) {
+    if constexpr (reg64_schedule_built(P, S, true)) {
+      if (c.variant - SE2W_KERNEL_REG64 == S) {
+        constexpr Reg64Schedule r = kReg64Schedules[S];
+        hipLaunchKernelGGL((reg64_kernel<P, r.ex, r.ey, r.waves, r.pf>), dim3(c.grid), dim3(r.ex * r.ey * 2), 0, c.stream,
+                           prm);
+        return true;
+      }
+    }
+    return launch_reg64_s<P, S + 1>(c, prm);
+  }
+  return false;
+}
 
 template <int P, bool FUSED, int S>
 bool launch_reg_s(const LaunchCfg &c, const StepParams<R> &prm) {
@@ -53,6 +70,9 @@ void launch_element_p(const LaunchCfg &c, const StepParams<R> &prm) {
       hipLaunchKernelGGL((line_kernel<P, kMode>), grid, dim3(kThreads), 0, c.stream, prm);
       return;
     }
+  }
+  if constexpr (kMode == MODE_F64 && FUSED) {
+    if (c.variant >= SE2W_KERNEL_REG64 && launch_reg64_s<P, 0>(c, prm)) return;
   }
   if constexpr (FUSED) {
     if (c.variant == SE2W_KERNEL_LDS_WIDE) {

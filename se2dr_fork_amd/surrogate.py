@@ -36,19 +36,19 @@ from . import scenarios as SC
 L = 4000.0
 VP0, RHO, SIGMA = 5000.0, 2700.0, 0.05
 GRID = 8
-F0, T0 = 1.0, 1.2
+F0 = 1.0  # Ricker peak frequency (Hz); the delay is 1.2 / f0
 DT, STRIDE, NSAMP = 5.0e-4, 14, 512
 POS_UNIT = 100.0
 RECEIVERS = np.array([[200.0, 2000.0], [3800.0, 2000.0], [2000.0, 200.0], [2000.0, 3800.0]])
 WIDTHS = [2 + GRID * GRID, 1024, 1024, 1024, len(RECEIVERS) * 2 * NSAMP]
 
 
-def instance(seed: int, mx=96, p=4):
+def instance(seed: int, mx=96, p=4, f0=F0):
     """(scenario, input vector) of the training/held-out instance `seed`."""
     rng = SC.Rng(40000 + seed)
     xs, ys = rng.uniform(0.1 * L, 0.9 * L), rng.uniform(0.1 * L, 0.9 * L)
     d = np.array([SIGMA * rng.normal() for _ in range(GRID * GRID)]).reshape(GRID, GRID)  # [block y][block x]
-    return scenario(xs, ys, d, mx, p), encode(xs, ys, d)
+    return scenario(xs, ys, d, mx, p, f0), encode(xs, ys, d)
 
 
 def encode(xs, ys, d) -> np.ndarray:
@@ -61,13 +61,13 @@ def decode(x: np.ndarray):
     return xs, ys, np.asarray(x[2:]).reshape(GRID, GRID)
 
 
-def scenario(xs, ys, d, mx=96, p=4) -> SC.Scenario:
+def scenario(xs, ys, d, mx=96, p=4, f0=F0) -> SC.Scenario:
     ey, ex = np.divmod(np.arange(mx * mx), mx)
     vp = VP0 * (1.0 + d[ey * GRID // mx, ex * GRID // mx])
     return SC.Scenario(name="surrogate", mx=mx, my=mx, p=p, scale=(L, L), shift=(0.0, 0.0), vp=vp,
                        vs=vp / np.sqrt(3.0), rho=np.full(mx * mx, RHO), src_xy=np.array([[xs, ys]]),
                        src_type=[SOURCE_FORCE], src_values=np.array([[0.0, 1.0e10, 0.0, 0.0]]),
-                       stf=[("ricker", T0, F0, 1.0)], rec_xy=RECEIVERS.copy(), nt=STRIDE * NSAMP, dt=DT)
+                       stf=[("ricker", 1.2 / f0, f0, 1.0)], rec_xy=RECEIVERS.copy(), nt=STRIDE * NSAMP, dt=DT)
 
 
 def seismograms(trace: np.ndarray) -> np.ndarray:
@@ -103,19 +103,19 @@ class Dataset:
     seconds: float       # wall time of the solves
 
 
-def generate(seeds: Sequence[int], mx=96, p=4, group=1152, log=None) -> Dataset:
-    pairs = [instance(s, mx, p) for s in seeds]
+def generate(seeds: Sequence[int], mx=96, p=4, group=1152, log=None, f0=F0) -> Dataset:
+    pairs = [instance(s, mx, p, f0) for s in seeds]
     t0 = time.perf_counter()
     y = solve([sc for sc, _ in pairs], group, log)
     return Dataset(x=np.array([x for _, x in pairs]), y=y, seconds=time.perf_counter() - t0)
 
 
 class Surrogate:
-    def __init__(self, widths: Optional[List[int]] = None, batch=256, seed=0, mx=96, p=4):
+    def __init__(self, widths: Optional[List[int]] = None, batch=256, seed=0, mx=96, p=4, f0=F0):
         self.widths = list(widths or WIDTHS)
         self.mlp = Mlp(self.widths, batch=batch, seed=seed)
         self.scale = 1.0
-        self.mx, self.p = mx, p
+        self.mx, self.p, self.f0 = mx, p, f0
         self.losses = None
 
     def fit(self, data: Dataset, epochs=200, lr=1e-3, seed=0, mirror=True, cosine=True) -> np.ndarray:
@@ -148,7 +148,7 @@ class Surrogate:
         if inside.any():
             y[inside] = self.predict(x[inside])
         if (~inside).any():
-            scs = [scenario(*decode(r), self.mx, self.p) for r in x[~inside]]
+            scs = [scenario(*decode(r), self.mx, self.p, self.f0) for r in x[~inside]]
             y[~inside] = solve(scs)
         return y, ~inside
 

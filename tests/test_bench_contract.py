@@ -18,6 +18,12 @@ def library_schedules():
 
 def test_schedule_table_matches_the_library():
     assert [tuple(s) for s in bench.REG_SCHEDULES] == library_schedules()
+    text = open(os.path.join(ROOT, "se2dr_fork_amd", "csrc", "hip", "se2w_kernels_reg64.hpp")).read()
+    body = re.search(r"kReg64Schedules\[\]\s*=\s*\{(.*?)\};", text, re.S).group(1)
+    assert [tuple(s) for s in bench.REG64_SCHEDULES] == [tuple(int(x) for x in m.split(","))
+                                                          for m in re.findall(r"\{([^{}]*)\}", body)]
+    header = open(os.path.join(ROOT, "include", "se2w.h")).read()
+    assert f"#define SE2W_KERNEL_REG64 {bench.KERNEL_REG64} " in header
 
 
 def test_kernel_signatures():

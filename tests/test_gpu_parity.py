@@ -65,6 +65,37 @@ def test_fp64_bit_identical_to_oracle(oracle, sc):
     np.testing.assert_array_equal(a, ao)
 
 
+FP64_VARIANTS = [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in range(4)]
+
+
+@pytest.mark.parametrize("p, mx, my", [(2, 9, 7), (3, 33, 5), (4, 41, 9), (4, 3, 2), (4, 70, 13), (4, 40, 1),
+                                        (4, 1, 17)])
+def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
+    """Every FP64 step kernel (LDS tile kernels, register kernel schedules)
+    runs the reference's operation order: receiver rows and the whole state
+    bit-identical to the oracle, on meshes of one tile, several tiles with
+    partial edge tiles, one element row and one element column."""
+    sc = SC.small(mx=mx, my=my, p=p, nt=30, seed=100 + p + mx, nsrc=2, nrec=5)
+    b0 = SC.build(sc)
+    pr = oracle.Problem(sc, dt=b0.dt, nt=b0.nsteps)
+    to, ro, uo, vo, ao = pr.run()
+    b0.solver.close()
+    ran = 0
+    for k in FP64_VARIANTS:
+        try:
+            b, times, rows, u, v, a = gpu_run(sc, kernel=k, chunks=2)
+        except S.Se2wError:
+            assert k >= S.KERNEL_REG64 and p > 4
+            continue
+        ran += 1
+        np.testing.assert_array_equal(rows, ro, err_msg=f"variant {k}")
+        np.testing.assert_array_equal(u, uo, err_msg=f"variant {k}")
+        np.testing.assert_array_equal(v, vo, err_msg=f"variant {k}")
+        np.testing.assert_array_equal(a, ao, err_msg=f"variant {k}")
+        b.solver.close()
+    assert ran == len(FP64_VARIANTS)
+
+
 @pytest.mark.parametrize("name", ["small_p3", "small_p4_force", "small_p8"])
 def test_fp64_golden_fixtures(name):
     z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
