@@ -112,9 +112,11 @@ def measured_traffic(mx, p, prec, signature):
 # library's variant order (SE2W_KERNEL_REG + k)
 REG_SCHEDULES = [(32, 4, 0, 2, 1), (16, 4, 0, 2, 1), (32, 2, 0, 2, 1), (32, 4, 0, 0, 2), (16, 4, 0, 4, 2),
                  (16, 8, 0, 4, 2), (16, 4, 0, 2, 2), (16, 4, 0, 0, 2)]
-# (elements in x, elements in y, minimum waves per SIMD, node-load placement)
-# of the FP64 register kernel's schedules (SE2W_KERNEL_REG64 + k)
-REG64_SCHEDULES = [(32, 4, 3, 0), (32, 4, 0, 0), (16, 8, 0, 4), (32, 4, 3, 4)]
+# (elements in x, elements in y, minimum waves per SIMD, node-load placement,
+# lanes per element) of the FP64 register kernel's schedules
+# (SE2W_KERNEL_REG64 + k)
+REG64_SCHEDULES = [(32, 4, 3, 0, 2), (32, 4, 0, 0, 2), (16, 8, 0, 4, 2), (32, 4, 3, 4, 2), (32, 4, 2, 0, 1),
+                   (64, 2, 2, 0, 1), (32, 4, 2, 4, 1)]
 KERNEL_REG64 = 12
 
 
@@ -129,8 +131,8 @@ def kernel_signature(variant, p, mode):
     if variant == 11:
         return f"tile_kernel<{p}, {mode}, true, 512>"
     if variant >= KERNEL_REG64:
-        ex, ey, w, pf = REG64_SCHEDULES[variant - KERNEL_REG64]
-        return f"reg64_kernel<{p}, {ex}, {ey}, {w}, {pf}>"
+        ex, ey, w, pf, lpe = REG64_SCHEDULES[variant - KERNEL_REG64]
+        return f"reg64_kernel<{p}, {ex}, {ey}, {w}, {pf}, {lpe}>"
     ex, ey, w, pf, lpe = REG_SCHEDULES[variant - 1]
     return f"reg_kernel<{p}, {mode}, true, {ex}, {ey}, {w}, {pf}, {lpe}>"
 
@@ -152,8 +154,9 @@ def ens_kernel_signature(variant, p, mode):
 def kernel_name(variant, p, mode):
     what = ("LDS-staged tile" if variant == 0 else "matrix-core contractions" if variant == 9 else
             "line contractions" if variant == 10 else "LDS-staged tile, 512 threads" if variant == 11 else
-            "FP64 lane pair per element in the reference's operation order, %dx%d elements" %
-            REG64_SCHEDULES[variant - KERNEL_REG64][:2] if variant >= KERNEL_REG64 else
+            "FP64 %s per element in the reference's operation order, %dx%d elements" %
+            (("lane", "lane pair")[REG64_SCHEDULES[variant - KERNEL_REG64][4] - 1],
+             *REG64_SCHEDULES[variant - KERNEL_REG64][:2]) if variant >= KERNEL_REG64 else
             "%s per element, %dx%d elements" % (("thread", "lane pair")[REG_SCHEDULES[variant - 1][4] - 1],
                                                 *REG_SCHEDULES[variant - 1][:2]))
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
@@ -189,6 +192,7 @@ def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, ker
     variant = sv.ctx.kernel()
     sv.close()
     out = {"wall_s": float(np.median(walls)), "wall_s_reps": walls, "event_ms": float(np.median(event_ms)),
+           "event_ms_reps": event_ms,
            "kernel_ms": float(kern[0]), "dt": b.dt, "variant": variant}
     if full_steps > 0:
         bf = SC.build(sc, precision=prec)
@@ -634,6 +638,7 @@ def main():
         "node_updates_per_s": {names[p]: r for p, r in rates.items()},
         "ms_per_step_by_precision": {names[p]: 1e3 * r["wall_s"] / args.steps for p, r in res.items()},
         "wall_s_reps": {names[p]: r["wall_s_reps"] for p, r in res.items()},
+        "gpu_event_ms_reps": {names[p]: r["event_ms_reps"] for p, r in res.items()},
         "kernel_ms": {names[p]: r["kernel_ms"] for p, r in res.items()},
         "kernels": {names[p]: kernel_signature(r["variant"], args.p, p) for p, r in res.items()},
         "roofline_by_precision": {

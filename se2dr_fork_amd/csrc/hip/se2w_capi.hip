@@ -3,6 +3,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
+#include <cstdlib>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -919,7 +921,17 @@ int se2w_reserve(se2w_ctx *c, int32_t nsteps) {
   if (nsteps < 1) return fail(SE2W_ERR_ARG, "nsteps must be >= 1");
   HIPCHK(hipSetDevice(c->device));
   if (int ierr = ensure_run_buffers(c, nsteps)) return ierr;
-  return ensure_stage_for(c, nsteps);
+  if (int ierr = ensure_stage_for(c, nsteps)) return ierr;
+  // One record download of the reserved length: the process's first
+  // device-to-host copy of this size sets up the runtime's copy path
+  // (measured at ~6.7 ms once per process), which would otherwise land in
+  // the first run's se2w_get_receivers.
+  if (c->nr > 0) {
+    const size_t bytes = (size_t)nsteps * c->nr * rec_stride(c->p) * real_size(c->mode);
+    HIPCHK(hipMemcpyAsync(c->h_stage, c->d_rec, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return SE2W_OK;
 }
 
 int se2w_get_counts(se2w_ctx *c, int32_t counts[4]) {
@@ -941,11 +953,24 @@ int se2w_get_reserved(se2w_ctx *c, int64_t caps[4]) {
 }
 
 int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
+  // SE2W_PHASE_TIMES=1 prints, per run, the host time of the argument checks
+  // and table upload, of the launches and the wait, and the GPU time between
+  // the run's first and last events (stderr).
+  static const bool trace = getenv("SE2W_PHASE_TIMES") != nullptr;
+  const auto t0 = std::chrono::steady_clock::now();
   int ierr = prepare_run(c, nsteps, dt, psi);
   if (ierr) return ierr;
+  const auto t1 = std::chrono::steady_clock::now();
   if (c->mode == SE2W_FP64) ierr = run_mode<MODE_F64>(c, nsteps, dt);
   else if (c->mode == SE2W_FP32) ierr = run_mode<MODE_F32>(c, nsteps, dt);
   else ierr = run_mode<MODE_F32K>(c, nsteps, dt);
+  if (trace) {
+    const auto t2 = std::chrono::steady_clock::now();
+    const double a = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    const double b = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    fprintf(stderr, "[se2w_run] %d steps: prepare %.3f ms, launch+wait %.3f ms, GPU events %.3f ms\n", nsteps, a, b,
+            c->last_ms);
+  }
   return ierr;
 }
 

@@ -37,8 +37,9 @@ namespace se2w {
 // to scratch) and the placement of the node-phase loads (0: after the force
 // exchange barrier, all slots at once; 4: in the node phase, CHUNK slots at
 // a time).
-struct Reg64Schedule { int ex, ey, waves, pf; };
-constexpr Reg64Schedule kReg64Schedules[] = {{32, 4, 3, 0}, {32, 4, 0, 0}, {16, 8, 0, 4}, {32, 4, 3, 4}};
+struct Reg64Schedule { int ex, ey, waves, pf, lpe; };
+constexpr Reg64Schedule kReg64Schedules[] = {{32, 4, 3, 0, 2}, {32, 4, 0, 0, 2}, {16, 8, 0, 4, 2}, {32, 4, 3, 4, 2},
+                                             {32, 4, 2, 0, 1}, {64, 2, 2, 0, 1}, {32, 4, 2, 4, 1}};
 constexpr int kNumReg64Schedules = sizeof(kReg64Schedules) / sizeof(kReg64Schedules[0]);
 // Degrees with a register-resident element (two (p+1)^2 arrays of doubles
 // per lane): 2 <= p <= 4; fused schedule only.
@@ -308,12 +309,250 @@ __device__ __forceinline__ void reg64_body(const StepParams<double> &prm, int ti
   }
 }
 
-template <int P, int EX, int EY, int WAVES, int PF>
-__global__ __launch_bounds__(EX * EY * 2) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+// One lane per element: the lane holds both displacement components and both
+// force accumulators of its element and forms every product, sum and force
+// term of the reference's loop itself (no cross-lane moves), in the same
+// order as the lane-pair body above.  A workgroup of EX x EY threads
+// processes one tile of (EX-1) x (EY-1) elements.
+template <int P, int EX, int EY, int PF>
+__device__ __forceinline__ void reg64s_body(const StepParams<double> &prm, int tile) {
+#pragma clang fp contract(off)
+  constexpr int NB = P + 1;
+  constexpr int TEX = EX - 1, TEY = EY - 1;
+  constexpr int NT = EX * EY;
+  __shared__ double2 redge[EY][EX][NB];  // right edge (i = p), j = 0..p
+  __shared__ double2 tedge[EY][EX][NB];  // top edge (j = p), i = 0..p
+  __shared__ double2 sf[TEY * P + 1][TEX * P + 1 + TEX + 1];
+  __shared__ double sW[NB * NB];
+  for (int k = threadIdx.x; k < NB * NB; k += NT) sW[k] = prm.w2[k];
+  __syncthreads();
+
+  const int tx = tile % prm.tiles_x, ty = tile / prm.tiles_x;
+  const int ex0 = tx * TEX, ey0 = ty * TEY;
+  const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
+  const int el = threadIdx.x;
+  const int lx = el % EX, ly = el / EX;
+  const int ex = ex0 - 1 + lx, ey = ey0 - 1 + ly;
+  const bool valid = ex >= 0 && ey >= 0 && ex < ex_end && ey < ey_end;
+  const bool owner = valid && lx >= 1 && ly >= 1;
+  const int ox1 = ex_end * P - 1 + (ex_end == prm.mx ? 1 : 0), oy1 = ey_end * P - 1 + (ey_end == prm.my ? 1 : 0);
+  const int OX = ox1 - ex0 * P + 1, OY = oy1 - ey0 * P + 1;
+  constexpr int LANES = NT < 64 ? NT : 64;
+  constexpr int NWAVES = NT / LANES;
+  static_assert(NWAVES * LANES == NT, "workgroup must be a multiple of its row width");
+  constexpr int RPW = (TEY * P + 1 + NWAVES - 1) / NWAVES;
+  constexpr int CPL = (TEX * P + 1 + LANES - 1) / LANES;
+  const int wave = threadIdx.x / LANES, lane = threadIdx.x % LANES;
+  // a row of elements that is a whole number of waves: the halo row's waves
+  // need only its top edge
+  constexpr bool HALO_WAVES = EX % 64 == 0;
+
+  double ux[NB][NB], uy[NB][NB];  // [j][i]
+  double fx[NB][NB], fy[NB][NB];
+  if (valid) {
+    const long base = (long)(ey * P) * prm.pitch + ex * P;
+    const double2 *ub = prm.u_in + base;
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        const double2 t = ub[(long)j * prm.pitch + i];
+        ux[j][i] = t.x;
+        uy[j][i] = t.y;
+      }
+    const int e = ex + ey * prm.mx;
+    const double J00 = 0.5 * prm.elem_dx[ex], J11 = 0.5 * prm.elem_dy[ey];
+    const double iJ00 = 1.0 / J00, iJ11 = 1.0 / J11, detJ = J00 * J11;
+    const double lam = prm.lambda[e], mu = prm.mu[e];
+    const double c11 = 2.0 * mu + lam;  // = c22; c12 = c21 = lambda, c33 = mu
+    using CD = const __attribute__((address_space(4))) double;
+    CD *D = (CD *)(uintptr_t)prm.D;
+    auto row_factors = [&](int qj, double *dny) {
+      double iy = iJ11;
+      asm volatile("" : "+v"(iy));
+#pragma unroll
+      for (int j = 0; j < NB; j++) dny[j] = iy * D[qj * NB + j];
+    };
+    // sxx, syy, sxy and -fac at q = (qi, qj), 1357-1402; dnx receives
+    // dN_dx on row qj.  e1 and the column half of e2 run in ascending j, e0
+    // and the row half of e2 in ascending i, e2 in ascending node order.
+    auto point_stress = [&](int qi, int qj, const double *dny, double *dnx, double &sxx, double &syy, double &sxy,
+                            double &nfac) {
+      double ix = iJ00;
+      asm volatile("" : "+v"(ix));
+#pragma unroll
+      for (int i = 0; i < NB; i++) dnx[i] = ix * D[qi * NB + i];
+      double e0 = 0.0, e1 = 0.0, e2 = 0.0;
+#pragma unroll
+      for (int j = 0; j < qj; j++) {
+        e1 = e1 + dny[j] * uy[j][qi];
+        e2 = e2 + dny[j] * ux[j][qi];
+      }
+      e1 = e1 + dny[qj] * uy[qj][qi];
+      const double tq = dny[qj] * ux[qj][qi];
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        e0 = e0 + dnx[i] * ux[qj][i];
+        double r = dnx[i] * uy[qj][i];
+        if (i == qi) r = r + tq;
+        e2 = e2 + r;
+      }
+#pragma unroll
+      for (int j = qj + 1; j < NB; j++) {
+        e1 = e1 + dny[j] * uy[j][qi];
+        e2 = e2 + dny[j] * ux[j][qi];
+      }
+      sxx = c11 * e0 + lam * e1;
+      syy = lam * e0 + c11 * e1;
+      sxy = mu * e2;
+      nfac = -(detJ * sW[qi + qj * NB]);
+    };
+    if (HALO_WAVES && __builtin_amdgcn_readfirstlane(ly) == 0) {
+      double gx[NB], gy[NB];  // top edge only
+#pragma unroll
+      for (int i = 0; i < NB; i++) gx[i] = gy[i] = 0.0;
+#pragma unroll
+      for (int qj = 0; qj < NB; qj++) {
+        double dny[NB];
+        row_factors(qj, dny);
+#pragma unroll
+        for (int qi = 0; qi < NB; qi++) {
+          double dnx[NB], sxx, syy, sxy, nfac;
+          point_stress(qi, qj, dny, dnx, sxx, syy, sxy, nfac);
+          if (qj == P) {
+#pragma unroll
+            for (int i = 0; i < NB; i++)
+              if (i != qi) {
+                gx[i] = gx[i] + nfac * (dnx[i] * sxx);
+                gy[i] = gy[i] + nfac * (dnx[i] * sxy);
+              }
+            gx[qi] = gx[qi] + nfac * (dnx[qi] * sxx + dny[qj] * sxy);
+            gy[qi] = gy[qi] + nfac * (dny[qj] * syy + dnx[qi] * sxy);
+          } else {
+            gx[qi] = gx[qi] + nfac * (dny[P] * sxy);
+            gy[qi] = gy[qi] + nfac * (dny[P] * syy);
+          }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NB; k++) tedge[ly][lx][k] = double2{gx[k], gy[k]};
+      redge[ly][lx][P] = double2{gx[P], gy[P]};
+    } else {
+#pragma unroll
+      for (int j = 0; j < NB; j++)
+#pragma unroll
+        for (int i = 0; i < NB; i++) fx[j][i] = fy[j][i] = 0.0;
+#pragma unroll
+      for (int qj = 0; qj < NB; qj++) {
+        double dny[NB];
+        row_factors(qj, dny);
+#pragma unroll
+        for (int qi = 0; qi < NB; qi++) {
+          double dnx[NB], sxx, syy, sxy, nfac;
+          point_stress(qi, qj, dny, dnx, sxx, syy, sxy, nfac);
+#pragma unroll
+          for (int i = 0; i < NB; i++)
+            if (i != qi) {
+              fx[qj][i] = fx[qj][i] + nfac * (dnx[i] * sxx);
+              fy[qj][i] = fy[qj][i] + nfac * (dnx[i] * sxy);
+            }
+          fx[qj][qi] = fx[qj][qi] + nfac * (dnx[qi] * sxx + dny[qj] * sxy);
+          fy[qj][qi] = fy[qj][qi] + nfac * (dny[qj] * syy + dnx[qi] * sxy);
+#pragma unroll
+          for (int j = 0; j < NB; j++)
+            if (j != qj) {
+              fx[j][qi] = fx[j][qi] + nfac * (dny[j] * sxy);
+              fy[j][qi] = fy[j][qi] + nfac * (dny[j] * syy);
+            }
+        }
+      }
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        redge[ly][lx][k] = double2{fx[k][P], fy[k][P]};
+        tedge[ly][lx][k] = double2{fx[P][k], fy[P][k]};
+      }
+    }
+  }
+  double2 pu[RPW][CPL], pv[RPW][CPL];
+  double pm[RPW][CPL];
+  auto node_loads = [&](int r, int cidx) {
+    const int lyn = min(wave + r * NWAVES, OY - 1), lxn = min(lane + cidx * LANES, OX - 1);
+    const long idx = (long)(ey0 * P + lyn) * prm.pitch + ex0 * P + lxn;
+    pu[r][cidx] = prm.u_in[idx];
+    pv[r][cidx] = prm.v[idx];
+    pm[r][cidx] = prm.mass[idx];
+  };
+  __syncthreads();
+  const TileLists tl = tile_lists(prm, tile);
+  constexpr int NSL = RPW * CPL, CHUNK = 8;
+  if constexpr (PF == 0) {
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int cidx = 0; cidx < CPL; cidx++) node_loads(r, cidx);
+  }
+  if (owner) {
+    const bool left = ex > 0, below = ey > 0;
+    const bool rb = ex == prm.mx - 1, tb = ey == prm.my - 1;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        if ((i < P || rb) && (j < P || tb)) {
+          const int lxn = (lx - 1) * P + i, lyn = (ly - 1) * P + j;
+          double gx = fx[j][i], gy = fy[j][i];
+          if (i == 0 && j == 0 && left && below) {  // (ex-1,ey-1), (ex,ey-1), (ex-1,ey), own
+            const double2 d = redge[ly - 1][lx - 1][P], b = tedge[ly - 1][lx][0], l = redge[ly][lx - 1][0];
+            gx = ((d.x + b.x) + l.x) + gx;
+            gy = ((d.y + b.y) + l.y) + gy;
+          } else if (i == 0 && left) {
+            const double2 l = redge[ly][lx - 1][j];
+            gx = l.x + gx;
+            gy = l.y + gy;
+          } else if (j == 0 && below) {
+            const double2 b = tedge[ly - 1][lx][i];
+            gx = b.x + gx;
+            gy = b.y + gy;
+          }
+          sf[lyn][lxn + lxn / P] = double2{gx, gy};
+        }
+      }
+    }
+  }
+  __syncthreads();
+  auto node_step = [&](int r, int cidx) {
+    const int lyn = wave + r * NWAVES, lxn = lane + cidx * LANES;
+    if (lyn < OY && lxn < OX) {
+      const int gx = ex0 * P + lxn, gy = ey0 * P + lyn;
+      const long idx = (long)gy * prm.pitch + gx;
+      const double2 f = sf[lyn][lxn + lxn / P];
+      node_update_core<P, MODE_F64>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
+                                    pm[r][cidx], double2{0.0, 0.0});
+    }
+  };
+  if constexpr (PF == 4) {
+#pragma unroll
+    for (int c0 = 0; c0 < NSL; c0 += CHUNK) {
+#pragma unroll
+      for (int sl = c0; sl < c0 + CHUNK && sl < NSL; sl++) node_loads(sl / CPL, sl % CPL);
+#pragma unroll
+      for (int sl = c0; sl < c0 + CHUNK && sl < NSL; sl++) node_step(sl / CPL, sl % CPL);
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < RPW; r++)
+#pragma unroll
+      for (int cidx = 0; cidx < CPL; cidx++) node_step(r, cidx);
+  }
+}
+
+template <int P, int EX, int EY, int WAVES, int PF, int LPE>
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
 void reg64_kernel(StepParams<double> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  reg64_body<P, EX, EY, PF>(prm, tile);
+  if constexpr (LPE == 2) reg64_body<P, EX, EY, PF>(prm, tile);
+  else reg64s_body<P, EX, EY, PF>(prm, tile);
 }
 
 }  // namespace se2w

@@ -26,8 +26,8 @@ bool launch_reg64_s(const LaunchCfg &c, const StepParams<R> &prm) {
     if constexpr (reg64_schedule_built(P, S, true)) {
       if (c.variant - SE2W_KERNEL_REG64 == S) {
         constexpr Reg64Schedule r = kReg64Schedules[S];
-        hipLaunchKernelGGL((reg64_kernel<P, r.ex, r.ey, r.waves, r.pf>), dim3(c.grid), dim3(r.ex * r.ey * 2), 0, c.stream,
-                           prm);
+        hipLaunchKernelGGL((reg64_kernel<P, r.ex, r.ey, r.waves, r.pf, r.lpe>), dim3(c.grid), dim3(r.ex * r.ey * r.lpe),
+                           0, c.stream, prm);
         return true;
       }
     }

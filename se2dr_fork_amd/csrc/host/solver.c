@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "host_internal.h"
 
@@ -179,25 +180,40 @@ int se2w_solver_reserve(SE2WSolver sv, int nsteps)
   return 0;
 }
 
+/* Monotonic wall clock in milliseconds (the SE2W_PHASE_TIMES trace). */
+static double now_ms(void)
+{
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return 1e3 * (double)ts.tv_sec + 1e-6 * (double)ts.tv_nsec;
+}
+
 int se2w_solver_advance(SE2WSolver sv, int nsteps, double dt)
 {
   const int nr = sv && sv->recv ? sv->recv->nr : 0;
-  double t;
+  double t, tp[4];
   int k, s, ierr;
   if (!sv || nsteps < 1) return host_fail(SE2W_HOST_ERR_ARG, "bad advance arguments");
   if ((ierr = grow(&sv->times, &sv->times_cap, (size_t)nsteps))) return ierr;
   if (sv->ns > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ns))) return ierr;
   if (nr > 0 && (ierr = grow(&sv->trace, &sv->trace_cap, (size_t)nsteps * nr * 7))) return ierr;
   t = sv->time;
+  tp[0] = now_ms();
   for (k = 0; k < nsteps; k++) {
     t = t + dt;
     sv->times[k] = t;
     for (s = 0; s < sv->ns; s++)
       if ((ierr = se2w_stf_evaluate(sv->stf[s], t, &sv->psi[(size_t)k * sv->ns + s]))) return ierr;
   }
+  tp[1] = now_ms();
   if ((ierr = se2w_run(sv->ctx, nsteps, dt, sv->psi))) return gpu_fail(ierr);
+  tp[2] = now_ms();
   if (nr > 0 && (ierr = se2w_get_receivers(sv->ctx, sv->trace))) return gpu_fail(ierr);
+  tp[3] = now_ms();
   if ((ierr = collect_uv(sv, nsteps))) return ierr;
+  if (getenv("SE2W_PHASE_TIMES"))
+    fprintf(stderr, "[se2w_solver_advance] %d steps: STF table %.3f ms, se2w_run %.3f ms, receiver rows %.3f ms, "
+                    "single-station rows %.3f ms\n", nsteps, tp[1] - tp[0], tp[2] - tp[1], tp[3] - tp[2], now_ms() - tp[3]);
   sv->time = t;
   sv->steps += nsteps;
   sv->last_n = nsteps;
