@@ -143,6 +143,18 @@ int se2w_get_receivers(se2w_ctx *ctx, double *trace);
  * state is not touched. */
 int se2w_assemble_forces(se2w_ctx *ctx, const double *u, double *f);
 
+/* se2w_assemble_forces without allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The same
+ * kernels and the same result as se2w_assemble_forces, with the staging and
+ * padded copies of u and f in a fixed 64 MiB device array that each GPU
+ * holds for the life of the process (FP64: meshes up to about 1.3 M nodes;
+ * FP32 modes: about 2.0 M; larger meshes return SE2W_ERR_UNSUPPORTED).  The
+ * array keeps the last call's contents; calls that overlap on one GPU (two
+ * host threads, or two contexts' streams) race for it and must be
+ * serialised by the caller.  It calls no hipMalloc or hipFree. */
+int se2w_assemble_forces_noalloc(se2w_ctx *ctx, const double *u, double *f);
+
 /* max, min and 2-norm over all dofs of u and of v, the quantities the
  * reference prints every 100 steps (VecMax/VecMin/VecNorm, se2wave.c:4747-4755):
  * out = {umax, umin, ul2, vmax, vmin, vl2}. */

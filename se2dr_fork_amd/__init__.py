@@ -80,6 +80,7 @@ def _declare(hip, host):
     hip.se2w_get_counts.argtypes = [v, _ip]
     hip.se2w_get_receivers.argtypes = [v, _dp]
     hip.se2w_assemble_forces.argtypes = [v, _dp, _dp]
+    hip.se2w_assemble_forces_noalloc.argtypes = [v, _dp, _dp]
     hip.se2w_norms.argtypes = [v, _dp]
     hip.se2w_set_profiling.argtypes = [v, i32]
     hip.se2w_get_timing.argtypes = [v, _dp, _dp]
@@ -626,6 +627,14 @@ class Context:
         u = self._sized("u", u, 2 * self.nnodes)
         f = np.zeros_like(u)
         _gchk(self._hip.se2w_assemble_forces(self._handle, _ptr(u), _ptr(f)))
+        return f
+
+    def assemble_forces_noalloc(self, u, f=None) -> np.ndarray:
+        """assemble_forces through the entry point that allocates no device
+        memory (se2w_assemble_forces_noalloc; meshes up to its scratch limit)."""
+        u = self._sized("u", u, 2 * self.nnodes)
+        f = np.zeros_like(u) if f is None else self._sized("f", f, 2 * self.nnodes)
+        _gchk(self._hip.se2w_assemble_forces_noalloc(self._handle, _ptr(u), _ptr(f)))
         return f
 
     def norms(self) -> np.ndarray:

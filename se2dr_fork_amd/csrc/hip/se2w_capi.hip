@@ -120,6 +120,44 @@ __global__ __launch_bounds__(256) void k_norms(const typename V2<R>::T *u, const
 
 size_t real_size(int mode) { return mode == SE2W_FP64 ? sizeof(double) : sizeof(float); }
 
+// Zero fill (a kernel rather than hipMemsetAsync, so that it can sit in a
+// captured stream).
+__global__ void k_zero(double *p, long n) {
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) p[i] = 0.0;
+}
+
+}  // namespace
+
+// Scratch of se2w_assemble_forces_noalloc: the host-layout staging copy of u
+// and f and the padded u and f of the element kernel.  Every GPU holds its
+// own copy of this array for the life of the process; host code uses only
+// the per-device address from hipGetSymbolAddress (the name itself is a
+// host-side stand-in).
+static constexpr size_t kScratchBytes = (size_t)64 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_dev[kMaxDevices] = {};
+
+// Device address of `scratch` on `device`, looked up once per device with
+// that device current (the caller has made it current).
+int scratch_address(int device, void **out) {
+  if (device < 0 || device >= kMaxDevices) return fail(SE2W_ERR_ARG, "device %d out of range", device);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(SE2W_ERR_HIP, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+    g_scratch_dev[device] = p;
+  }
+  *out = g_scratch_dev[device];
+  return SE2W_OK;
+}
+
 }  // namespace
 
 struct se2w_ctx {
@@ -551,6 +589,10 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
   int ierr = SE2W_OK;
   hipError_t he = hipSetDevice(c->device);
   if (he != hipSuccess) { ierr = fail(SE2W_ERR_HIP, "hipSetDevice: %s", hipGetErrorString(he)); goto bad; }
+  {  // the device's scratch address, looked up once per device at set-up
+    void *sp = nullptr;
+    if ((ierr = scratch_address(c->device, &sp))) goto bad;
+  }
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
     ierr = fail(SE2W_ERR_HIP, "stream/event creation failed");
@@ -1088,6 +1130,49 @@ int se2w_assemble_forces(se2w_ctx *c, const double *u, double *f) {
                        (const float2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(f, tmp, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return SE2W_OK;
+}
+
+int se2w_assemble_forces_noalloc(se2w_ctx *c, const double *u, double *f) {
+  if (!c || !u || !f) return fail(SE2W_ERR_ARG, "null argument");
+  if (!c->have_tables || !c->have_cells) return fail(SE2W_ERR_STATE, "set tables and cells first");
+  auto al = [](size_t x) { return (x + 255) / 256 * 256; };
+  const long n = (long)c->nx * c->ny;
+  const size_t stage = sizeof(double) * 2 * (size_t)n;
+  const size_t field = (size_t)c->pitch * c->ny * 2 * real_size(c->mode);
+  const size_t need = al(stage) + 2 * al(field);
+  if (need > kScratchBytes)
+    return fail(SE2W_ERR_UNSUPPORTED,
+                "se2w_assemble_forces_noalloc: a mesh of %ld nodes needs %zu bytes of scratch, above its %zu-byte limit "
+                "(use se2w_assemble_forces)",
+                n, need, kScratchBytes);
+  HIPCHK(hipSetDevice(c->device));
+  void *base = nullptr;
+  if (int ierr = scratch_address(c->device, &base)) return ierr;
+  double *tmp = (double *)base;
+  void *u_dev = (char *)base + al(stage);
+  void *f_dev = (char *)base + al(stage) + al(field);
+  HIPCHK(hipMemcpyAsync(tmp, u, stage, hipMemcpyHostToDevice, c->stream));
+  // the scratch keeps the previous call's contents: clear f, which the
+  // element kernel writes only at the mesh's nodes (not the row padding)
+  hipLaunchKernelGGL(k_zero, dim3(grid_for((long)(field / sizeof(double)))), dim3(256), 0, c->stream, (double *)f_dev,
+                     (long)(field / sizeof(double)));
+  if (c->mode == SE2W_FP64)
+    hipLaunchKernelGGL(k_from_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (double2 *)u_dev, c->nx,
+                       c->ny, c->pitch);
+  else
+    hipLaunchKernelGGL(k_from_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, tmp, (float2 *)u_dev, c->nx,
+                       c->ny, c->pitch);
+  if (int ierr = enqueue_forces(c, u_dev, f_dev)) return ierr;
+  if (c->mode == SE2W_FP64)
+    hipLaunchKernelGGL(k_to_dmda<double>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const double2 *)f_dev,
+                       (const double2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
+  else
+    hipLaunchKernelGGL(k_to_dmda<float>, dim3(grid_for(n)), dim3(256), 0, c->stream, (const float2 *)f_dev,
+                       (const float2 *)nullptr, tmp, c->nx, c->ny, c->pitch);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(f, tmp, stage, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return SE2W_OK;
 }
