@@ -123,7 +123,7 @@ KERNEL_REG64 = 12
 def kernel_signature(variant, p, mode):
     """The step kernel's template signature as rocprofv3 prints it."""
     if variant == 0:
-        return f"tile_kernel<{p}, {mode}, true>"
+        return f"tile_kernel<{p}, {mode}, true, 256>"
     if variant == 9:
         return f"mfma_kernel<{p}, {mode}>"
     if variant == 10:
@@ -561,7 +561,7 @@ def main():
     ap.add_argument("--no-beyond-cache", action="store_true",
                     help="skip the leg at twice the mesh width, whose state exceeds the Infinity Cache")
     ap.add_argument("--kernel", choices=["auto", "lds", "reg"], default="auto",
-                    help="element-kernel variant of the FP32 modes (FP64 always uses the LDS kernel)")
+                    help="element-kernel variant of the FP32 modes (FP64 takes its autotuned variant)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "the benchmark runs on one GPU (the step would shard by element-aligned "

@@ -168,6 +168,13 @@ struct se2w_ctx {
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   void *U[2] = {nullptr, nullptr}, *C[2] = {nullptr, nullptr};
   void *V = nullptr, *A = nullptr, *F = nullptr, *Utmp = nullptr;
+  // The next run's half-step velocity and (FP32+Kahan) compensation, written
+  // by a run's last step together with the next run's predicted displacement
+  // in U[cur ^ 1]; valid (pred_ok) until the state is written from the host,
+  // for a next run with the same dt (pred_dt).
+  void *Vn = nullptr, *Cn = nullptr;
+  bool pred_ok = false;
+  double pred_dt = 0.0;
   void *mass = nullptr, *lambda = nullptr, *mu = nullptr, *edx = nullptr, *edy = nullptr, *D = nullptr,
        *w2 = nullptr;
   int cur = 0;
@@ -362,6 +369,8 @@ StepParams<typename RealOf<MODE>::T> make_params(se2w_ctx *c, int step, double d
   p.f = (R2 *)c->F;
   p.c_in = (const R2 *)c->C[c->cur];
   p.c_out = (R2 *)c->C[c->cur ^ 1];
+  p.v_next = (R2 *)c->Vn;
+  p.c_next = (R2 *)c->Cn;
   p.mass = (const R *)c->mass;
   p.lambda = (const R *)c->lambda;
   p.mu = (const R *)c->mu;
@@ -462,6 +471,34 @@ struct ScopedEvents {
   hipEvent_t &operator[](size_t i) { return ev[i]; }
 };
 
+// Opens a run on stream s: the step kernels start from the predicted state
+// (u*, v*) of the run's first step.  The previous run's last step left it in
+// U[cur ^ 1], Vn (and Cn) when that run had the same dt and the state has not
+// been written since; otherwise the predictor kernel forms it from the plain
+// state (U[cur], V, A, C).  Either way U[cur], V and C then hold it.
+template <int MODE>
+void open_run(se2w_ctx *c, double dt, hipStream_t s) {
+  if (c->pred_ok && c->pred_dt == dt) {
+    std::swap(c->V, c->Vn);
+    if (c->mode == SE2W_FP32_KAHAN) {
+      std::swap(c->C[0], c->Cn);
+      c->C[1] = c->C[0];
+    }
+  } else {
+    StepParams<typename RealOf<MODE>::T> prm = make_params<MODE>(c, 0, dt, 0);
+    launch_predictor<MODE>(prm, (long)c->pitch * c->ny, s);
+  }
+  c->cur ^= 1;
+  c->pred_ok = false;  // until the run's last step has written the next one
+}
+
+// Closes a run whose last step read U[cur]: that buffer keeps the plain
+// displacement, and the next run's predicted state is in place.
+void close_run(se2w_ctx *c, double dt) {
+  c->pred_ok = true;
+  c->pred_dt = dt;
+}
+
 template <int MODE>
 int run_mode(se2w_ctx *c, int nsteps, double dt) {
   using R = typename RealOf<MODE>::T;
@@ -473,20 +510,17 @@ int run_mode(se2w_ctx *c, int nsteps, double dt) {
   if (c->profiling)
     if (int ierr = evs.create(fused ? 2 : (size_t)nsteps * 3)) return ierr;
   HIPCHK(hipEventRecord(c->ev0, c->stream));
-  {
-    StepParams<R> prm = make_params<MODE>(c, 0, dt, 0);
-    launch_predictor<MODE>(prm, (long)c->pitch * c->ny, c->stream);
-    HIPCHK(hipGetLastError());
-    c->cur ^= 1;
-  }
+  open_run<MODE>(c, dt, c->stream);
+  HIPCHK(hipGetLastError());
   if (c->profiling && fused) HIPCHK(hipEventRecord(evs[0], c->stream));
   for (int k = 0; k < nsteps; k++) {
     StepParams<R> prm = make_params<MODE>(c, k, dt, k == nsteps - 1);
     launch_step<MODE>(launch_cfg(c), prm, (c->profiling && !fused) ? &evs[(size_t)k * 3] : nullptr);
-    c->cur ^= 1;
+    if (k < nsteps - 1) c->cur ^= 1;
   }
   if (c->profiling && fused) HIPCHK(hipEventRecord(evs[1], c->stream));
   HIPCHK(hipGetLastError());
+  close_run(c, dt);
   HIPCHK(hipEventRecord(c->ev1, c->stream));
   if (int ierr = ensure_stage(c)) return ierr;
   HIPCHK(hipEventSynchronize(c->ev1));
@@ -602,15 +636,16 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
     const size_t rs = real_size(c->mode);
     const size_t field = (size_t)c->pitch * c->ny * 2 * rs;
     if ((ierr = dev_alloc(&c->U[0], field)) || (ierr = dev_alloc(&c->U[1], field)) || (ierr = dev_alloc(&c->V, field)) ||
-        (ierr = dev_alloc(&c->A, field)))
+        (ierr = dev_alloc(&c->A, field)) || (ierr = dev_alloc(&c->Vn, field)))
       goto bad;
     if (c->mode == SE2W_FP32_KAHAN) {
       // One compensation buffer, updated in place (C[1] aliases C[0]): a
       // node's compensation is read and written only by the thread that
       // updates the node, unlike u, whose halo other tiles read.
-      if ((ierr = dev_alloc(&c->C[0], field))) goto bad;
+      if ((ierr = dev_alloc(&c->C[0], field)) || (ierr = dev_alloc(&c->Cn, field))) goto bad;
       c->C[1] = c->C[0];
       hipMemsetAsync(c->C[0], 0, field, c->stream);
+      hipMemsetAsync(c->Cn, 0, field, c->stream);
     }
     if (c->schedule == SE2W_SCHED_SPLIT) {
       if ((ierr = dev_alloc(&c->F, field))) goto bad;
@@ -620,6 +655,7 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
     hipMemsetAsync(c->U[1], 0, field, c->stream);
     hipMemsetAsync(c->V, 0, field, c->stream);
     hipMemsetAsync(c->A, 0, field, c->stream);
+    hipMemsetAsync(c->Vn, 0, field, c->stream);
     if ((ierr = dev_alloc((void **)&c->d_partial, sizeof(double) * 6 * 1024))) goto bad;
   }
   {  // empty source / receiver lists
@@ -641,7 +677,7 @@ int se2w_destroy(se2w_ctx *c) {
   if (!c) return SE2W_OK;
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->C[1] == c->C[0]) c->C[1] = nullptr;  // aliased compensation buffer
-  void **bufs[] = {&c->U[0], &c->U[1], &c->C[0], &c->C[1], &c->V, &c->A, &c->F, &c->Utmp, &c->mass, &c->lambda,
+  void **bufs[] = {&c->U[0], &c->U[1], &c->C[0], &c->C[1], &c->V, &c->A, &c->Vn, &c->Cn, &c->F, &c->Utmp, &c->mass, &c->lambda,
                    &c->mu, &c->edx, &c->edy, &c->D, &c->w2, &c->d_src_vals, &c->d_psi, &c->d_rec, &c->d_erec};
   for (void **b : bufs) dev_free(*b);
   int **ibufs[] = {&c->d_src_elem, &c->d_tsrc_ptr, &c->d_tsrc_idx, &c->d_rec_elem, &c->d_rec_gll,
@@ -841,6 +877,7 @@ int se2w_set_state(se2w_ctx *c, const double *u, const double *v, const double *
   if (u && c->mode == SE2W_FP32_KAHAN)
     HIPCHK(hipMemsetAsync(c->C[c->cur], 0, (size_t)c->pitch * c->ny * 2 * sizeof(float), c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  c->pred_ok = false;
   return SE2W_OK;
 }
 
@@ -1275,7 +1312,6 @@ int size_bucket(int n) {
 // Copy of a context's state (u, v, a and the compensation of the current
 // buffer) kept on the device while trial runs overwrite it.
 struct Snapshot {
-  void *src[4] = {nullptr, nullptr, nullptr, nullptr};
   void *copy[4] = {nullptr, nullptr, nullptr, nullptr};
   size_t bytes = 0;
   int cur = 0, ns = 0, nr = 0;
@@ -1289,7 +1325,6 @@ int snapshot_take(se2w_ctx *c, Snapshot &sn) {
   sn.bytes = (size_t)c->pitch * c->ny * 2 * real_size(c->mode);
   void *srcs[4] = {c->U[c->cur], c->V, c->A, c->C[c->cur]};
   for (int f = 0; f < 4; f++) {
-    sn.src[f] = srcs[f];
     if (!srcs[f]) continue;
     int ierr = dev_alloc(&sn.copy[f], sn.bytes);
     if (ierr) { snapshot_free(sn); return ierr; }
@@ -1304,8 +1339,12 @@ int snapshot_take(se2w_ctx *c, Snapshot &sn) {
 
 int snapshot_restore(se2w_ctx *c, Snapshot &sn) {
   c->cur = sn.cur;
+  c->pred_ok = false;  // the restored plain state; the next run forms its predictor
+  // the runs since the snapshot may have exchanged V and C with Vn and Cn:
+  // restore into the buffers that hold the plain state now
+  void *dsts[4] = {c->U[c->cur], c->V, c->A, c->C[c->cur]};
   for (int f = 0; f < 4; f++)
-    if (sn.copy[f]) HIPCHK(hipMemcpyAsync(sn.src[f], sn.copy[f], sn.bytes, hipMemcpyDeviceToDevice, c->stream));
+    if (sn.copy[f]) HIPCHK(hipMemcpyAsync(dsts[f], sn.copy[f], sn.bytes, hipMemcpyDeviceToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   snapshot_free(sn);
   c->ns = sn.ns;
@@ -1483,15 +1522,10 @@ int ens_launch_steps(se2w_ensemble *e, int nsteps) {
 
 template <int MODE>
 int ens_run_mode(se2w_ensemble *e, int nsteps, const double *dt) {
-  using R = typename RealOf<MODE>::T;
   HIPCHK(hipEventRecord(e->ev0, e->stream));
-  // predictor of every member, then the step parameters of both parities
-  for (size_t i = 0; i < e->members.size(); i++) {
-    se2w_ctx *c = e->members[i];
-    StepParams<R> prm = make_params<MODE>(c, 0, dt[i], 0);
-    launch_predictor<MODE>(prm, (long)c->pitch * c->ny, e->stream);
-    c->cur ^= 1;
-  }
+  // every member's predicted state (carried over or formed), then the step
+  // parameters of both parities
+  for (size_t i = 0; i < e->members.size(); i++) open_run<MODE>(e->members[i], dt[i], e->stream);
   HIPCHK(hipGetLastError());
   if (int ierr = ens_step_params<MODE>(e, dt)) return ierr;
   HIPCHK(hipEventRecord(e->ev1, e->stream));
@@ -1514,8 +1548,10 @@ int ens_run_mode(se2w_ensemble *e, int nsteps, const double *dt) {
     if (int ierr = ens_launch_steps<MODE>(e, nsteps)) return ierr;
   }
   HIPCHK(hipEventRecord(e->ev2, e->stream));
-  for (se2w_ctx *c : e->members) {
-    c->cur ^= (nsteps & 1);
+  for (size_t i = 0; i < e->members.size(); i++) {
+    se2w_ctx *c = e->members[i];
+    c->cur ^= ((nsteps - 1) & 1);  // the last step's input: the plain displacement
+    close_run(c, dt[i]);
     if (int ierr = ensure_stage(c)) return ierr;
   }
   HIPCHK(hipEventSynchronize(e->ev2));

@@ -72,6 +72,8 @@ template <typename R> struct StepParams {
   R2 *f;           // split mode: internal force between the two kernels
   const R2 *c_in;  // Kahan compensation (MODE_F32K)
   R2 *c_out;
+  R2 *v_next;      // last step: the next run's half-step velocity (null: not formed)
+  R2 *c_next;      // last step, MODE_F32K: the next run's compensation
   const R *mass;   // MODE_F64: lumped mass M; FP32 modes: 1/M (per node, padded)
   const R *lambda, *mu;
   const R *elem_dx, *elem_dy;
@@ -91,7 +93,12 @@ template <typename R> struct StepParams {
   int ner;
   int step;           // step index inside the current run
   R dt, hdt, hdt2;    // dt, 0.5*dt, 0.5*dt*dt (formed in FP64 on the host)
-  int last;           // 1: keep the corrected state (no predictor), write a
+  // Last step of a run: v and a receive the corrected state of the step, u_in
+  // keeps its displacement; with v_next set, the predictor of the next run's
+  // first step is applied as on every other step, into u_out, v_next and
+  // c_next, so that a next run with the same dt needs no predictor pass;
+  // without, u_out receives a copy of the displacement.
+  int last;
 };
 
 // XCD-aware tile order: hardware places workgroup b on XCD (b mod 8).  Give
@@ -307,17 +314,21 @@ __device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, c
   const double ax = fx / mass, ay = fy / mass;
   const double vx = vv.x + prm.hdt * ax, vy = vv.y + prm.hdt * ay;
   record_receivers<P, double>(prm, tl, gx, gy, ux, uy, vx, vy, ax, ay);
+  double2 *vdst = prm.v;
   if (prm.last) {
-    prm.u_out[idx] = double2{ux, uy};
     prm.v[idx] = double2{vx, vy};
     prm.a[idx] = double2{ax, ay};
-    return;
+    if (!prm.v_next) {
+      prm.u_out[idx] = double2{ux, uy};
+      return;
+    }
+    vdst = prm.v_next;
   }
   double nux = ux + prm.dt * vx, nuy = uy + prm.dt * vy;
   nux = nux + prm.hdt2 * ax;
   nuy = nuy + prm.hdt2 * ay;
   prm.u_out[idx] = double2{nux, nuy};
-  prm.v[idx] = double2{vx + prm.hdt * ax, vy + prm.hdt * ay};
+  vdst[idx] = double2{vx + prm.hdt * ax, vy + prm.hdt * ay};
 }
 
 // FP32: same sequence with fused multiply-adds; MODE_F32K accumulates the
@@ -329,12 +340,18 @@ __device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, co
   const float ax = fx * minv, ay = fy * minv;
   const float vx = fmaf(prm.hdt, ax, vv.x), vy = fmaf(prm.hdt, ay, vv.y);
   record_receivers<P, float>(prm, tl, gx, gy, ux, uy, vx, vy, ax, ay);
+  float2 *vdst = prm.v;
+  [[maybe_unused]] float2 *cdst = prm.c_out;
   if (prm.last) {
-    prm.u_out[idx] = float2{ux, uy};
     prm.v[idx] = float2{vx, vy};
     prm.a[idx] = float2{ax, ay};
-    if constexpr (MODE == MODE_F32K) prm.c_out[idx] = cc;
-    return;
+    if (!prm.v_next) {
+      prm.u_out[idx] = float2{ux, uy};
+      if constexpr (MODE == MODE_F32K) prm.c_out[idx] = cc;
+      return;
+    }
+    vdst = prm.v_next;  // the compensation of u stays in c_in (c_out aliases it)
+    cdst = prm.c_next;
   }
   float nux, nuy;
   if constexpr (MODE == MODE_F32K) {
@@ -345,13 +362,13 @@ __device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, co
     const float m0 = nux, m1 = nuy;
     y = fmaf(prm.hdt2, ax, -c.x); t = m0 + y; c.x = (t - m0) - y; nux = t;
     y = fmaf(prm.hdt2, ay, -c.y); t = m1 + y; c.y = (t - m1) - y; nuy = t;
-    prm.c_out[idx] = c;
+    cdst[idx] = c;
   } else {
     nux = fmaf(prm.hdt2, ax, fmaf(prm.dt, vx, ux));
     nuy = fmaf(prm.hdt2, ay, fmaf(prm.dt, vy, uy));
   }
   prm.u_out[idx] = float2{nux, nuy};
-  prm.v[idx] = float2{fmaf(prm.hdt, ax, vx), fmaf(prm.hdt, ay, vy)};
+  vdst[idx] = float2{fmaf(prm.hdt, ax, vx), fmaf(prm.hdt, ay, vy)};
 }
 
 // ---------------------------------------------------------------------------
@@ -680,7 +697,10 @@ __global__ __launch_bounds__(kThreads) void update_kernel(StepParams<typename Re
 }
 
 // Predictor that opens a run from a plain state (u_n, v_n, a_n):
-// u* = u + dt v + (0.5 dt^2) a;  v* = v + (0.5 dt) a   (se2wave.c:4726-4730).
+// u* = u + dt v + (0.5 dt^2) a;  v* = v + (0.5 dt) a   (se2wave.c:4726-4730),
+// with the operations of the step kernels' own predictor (FP64: each product
+// and sum rounded; FP32 modes: fused multiply-adds), so that a run split into
+// chunks gives the bits of the unsplit run.
 template <int MODE>
 __global__ __launch_bounds__(kThreads) void predictor_kernel(StepParams<typename RealOf<MODE>::T> prm) {
 #pragma clang fp contract(off)
@@ -697,12 +717,15 @@ __global__ __launch_bounds__(kThreads) void predictor_kernel(StepParams<typename
     if constexpr (MODE == MODE_F32K) {
       R2 c = prm.c_in[idx];
       R y, t;
-      y = prm.dt * v.x - c.x; t = u.x + y; c.x = (t - u.x) - y; nux = t;
-      y = prm.dt * v.y - c.y; t = u.y + y; c.y = (t - u.y) - y; nuy = t;
+      y = fmaf(prm.dt, v.x, -c.x); t = u.x + y; c.x = (t - u.x) - y; nux = t;
+      y = fmaf(prm.dt, v.y, -c.y); t = u.y + y; c.y = (t - u.y) - y; nuy = t;
       R m0 = nux, m1 = nuy;
-      y = prm.hdt2 * a.x - c.x; t = m0 + y; c.x = (t - m0) - y; nux = t;
-      y = prm.hdt2 * a.y - c.y; t = m1 + y; c.y = (t - m1) - y; nuy = t;
+      y = fmaf(prm.hdt2, a.x, -c.x); t = m0 + y; c.x = (t - m0) - y; nux = t;
+      y = fmaf(prm.hdt2, a.y, -c.y); t = m1 + y; c.y = (t - m1) - y; nuy = t;
       prm.c_out[idx] = c;
+    } else if constexpr (MODE == MODE_F32) {
+      nux = fmaf(prm.hdt2, a.x, fmaf(prm.dt, v.x, u.x));
+      nuy = fmaf(prm.hdt2, a.y, fmaf(prm.dt, v.y, u.y));
     } else {
       nux = u.x + prm.dt * v.x;
       nuy = u.y + prm.dt * v.y;
@@ -710,7 +733,8 @@ __global__ __launch_bounds__(kThreads) void predictor_kernel(StepParams<typename
       nuy = nuy + prm.hdt2 * a.y;
     }
     prm.u_out[idx] = R2{nux, nuy};
-    prm.v[idx] = R2{v.x + prm.hdt * a.x, v.y + prm.hdt * a.y};
+    if constexpr (MODE == MODE_F64) prm.v[idx] = R2{v.x + prm.hdt * a.x, v.y + prm.hdt * a.y};
+    else prm.v[idx] = R2{fmaf(prm.hdt, a.x, v.x), fmaf(prm.hdt, a.y, v.y)};
   }
 }
 

@@ -27,8 +27,11 @@ def test_schedule_table_matches_the_library():
 
 
 def test_kernel_signatures():
-    assert bench.kernel_signature(0, 4, 2) == "tile_kernel<4, 2, true>"
+    # as rocprofv3 names the kernels: the LDS tile kernel with its thread count
+    assert bench.kernel_signature(0, 4, 2) == "tile_kernel<4, 2, true, 256>"
     assert bench.kernel_signature(11, 8, 1) == "tile_kernel<8, 1, true, 512>"
+    ex, ey, w, pf, lpe = bench.REG64_SCHEDULES[2]
+    assert bench.kernel_signature(bench.KERNEL_REG64 + 2, 4, 0) == f"reg64_kernel<4, {ex}, {ey}, {w}, {pf}, {lpe}>"
     ex, ey, w, pf, lpe = bench.REG_SCHEDULES[6]
     assert bench.kernel_signature(7, 4, 2) == f"reg_kernel<4, 2, true, {ex}, {ey}, {w}, {pf}, {lpe}>"
     assert bench.ens_kernel_signature(4, 6, 1).startswith("ens_reg_kernel<6, 1, ")

@@ -150,16 +150,63 @@ def test_split_schedule_equals_fused(oracle):
         assert rel_l2(u2, u1) < 1e-5
 
 
-def test_chunked_advance_equals_single_advance():
-    """Runs restart from the plain state (u, v, a); chunking must not change bits."""
-    sc = SC.small(mx=6, my=5, p=3, nt=60)
-    _, t1, r1, u1, v1, a1 = gpu_run(sc)
-    _, t2, r2, u2, v2, a2 = gpu_run(sc, chunks=7)
+@pytest.mark.parametrize("prec, schedule", [(S.FP64, S.SCHED_FUSED), (S.FP32, S.SCHED_FUSED),
+                                             (S.FP32_KAHAN, S.SCHED_FUSED), (S.FP64, S.SCHED_SPLIT),
+                                             (S.FP32_KAHAN, S.SCHED_SPLIT)])
+@pytest.mark.parametrize("mesh", ["one tile", "many tiles"])
+def test_chunked_advance_equals_single_advance(prec, schedule, mesh):
+    """A run's last step writes the plain state (u, v, a) and, for a next run
+    with the same dt, that run's predicted state with the operations of the
+    predictor kernel: chunking changes no bits in any mode or schedule."""
+    sc = SC.small(mx=6, my=5, p=3, nt=60) if mesh == "one tile" else SC.small(mx=41, my=37, p=4, nt=45, seed=13)
+    _, t1, r1, u1, v1, a1 = gpu_run(sc, precision=prec, schedule=schedule)
+    _, t2, r2, u2, v2, a2 = gpu_run(sc, precision=prec, schedule=schedule, chunks=7)
     np.testing.assert_array_equal(t1, t2)
     np.testing.assert_array_equal(r1, r2)
     np.testing.assert_array_equal(u1, u2)
     np.testing.assert_array_equal(v1, v2)
     np.testing.assert_array_equal(a1, a2)
+
+
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32, S.FP32_KAHAN])
+def test_run_after_a_dt_change_or_a_state_write_predicts_from_the_plain_state(prec):
+    """The predicted state a run leaves behind serves only a next run with
+    the same dt on an unwritten state.  A run with another dt, or after
+    set_state, must predict from the plain state: it equals a fresh context
+    given that plain state (bit for bit; after the dt change FP32+Kahan's
+    carried compensation, which a fresh context starts at zero, allows
+    round-off, far below the O(1) error of a stale prediction)."""
+    sc = SC.small(mx=20, my=11, p=3, nt=40, seed=21)
+    rng = np.random.default_rng(5)
+
+    def fresh(state, dt, psi, n):
+        b = SC.build(sc, precision=prec)
+        b.solver.ctx.set_state(*state)
+        b.solver.ctx.run(n, dt, psi)
+        return (*b.solver.ctx.get_state(), b.solver.ctx.receivers())
+
+    b = SC.build(sc, precision=prec)
+    ctx = b.solver.ctx
+    ns = ctx.counts()[0]
+    assert ns > 0
+    psi1, psi2 = rng.standard_normal(15 * ns), rng.standard_normal(10 * ns)
+    ctx.run(15, b.dt, psi1)
+    plain = ctx.get_state()
+    ctx.run(10, 0.5 * b.dt, psi2)  # another dt
+    got = (*ctx.get_state(), ctx.receivers())
+    want = fresh(plain, 0.5 * b.dt, psi2, 10)
+    for g, w in zip(got, want):
+        if prec == S.FP32_KAHAN:
+            assert rel_l2(g, w) < 1e-6
+        else:
+            np.testing.assert_array_equal(g, w)
+    new = tuple(x + 1e-4 * rng.standard_normal(x.size) for x in got[:3])
+    ctx.set_state(*new)  # a state written between two runs with the same dt
+    ctx.run(10, 0.5 * b.dt, psi2)
+    got = (*ctx.get_state(), ctx.receivers())
+    want = fresh(new, 0.5 * b.dt, psi2, 10)
+    for g, w in zip(got, want):
+        np.testing.assert_array_equal(g, w)
 
 
 def test_state_roundtrip_and_restart(oracle):
