@@ -229,7 +229,8 @@ int se2w_ensemble_autotune(se2w_ensemble *ens, int32_t steps, int32_t reps, int3
 int se2w_ensemble_set_graph(se2w_ensemble *ens, int32_t on);
 /* Last run: whole run (predictors included) and mean time per batched step. */
 int se2w_ensemble_get_timing(se2w_ensemble *ens, double *run_ms, double *step_ms);
-/* Tiles of all members and workgroups of the persistent batched launch. */
+/* Tiles of all members and workgroups of the batched launch (one per tile,
+ * rounded up to a multiple of 8). */
 int se2w_ensemble_get_launch(se2w_ensemble *ens, int32_t *tiles, int32_t *grid);
 /* Member i of the ensemble. */
 int se2w_ensemble_get_member(se2w_ensemble *ens, int32_t i, se2w_ctx **member);

@@ -720,7 +720,7 @@ class Ensemble:
         return a.value, b.value
 
     def launch(self):
-        """(tiles of all members, workgroups of the persistent launch)."""
+        """(tiles of all members, workgroups of the batched launch: one per tile)."""
         a, b = C.c_int32(), C.c_int32()
         _gchk(self._hip.se2w_ensemble_get_launch(self.h, C.byref(a), C.byref(b)))
         return a.value, b.value

@@ -1444,7 +1444,7 @@ struct se2w_ensemble {
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr;
   void *d_prms = nullptr;
   int *d_tile_off = nullptr;
-  int total = 0, grid = 0, cus = 0;
+  int total = 0, grid = 0;
   bool use_graph = true;
   hipGraphExec_t exec = nullptr;
   int exec_steps = 0;
@@ -1465,8 +1465,7 @@ bool ens_variant_built(const se2w_ensemble *e, int variant) {
 }
 
 // Sets the members' tiling for `variant`, the global tile offsets and the
-// persistent grid (as many workgroups as fit the device at once, at most one
-// per tile, a multiple of 8).
+// grid: one workgroup per tile, rounded up to a multiple of 8.
 int ens_set_variant(se2w_ensemble *e, int variant) {
   if (!ens_variant_built(e, variant))
     return fail(SE2W_ERR_UNSUPPORTED, "kernel variant %d is not built for ensembles at p = %d", variant, e->p);
@@ -1476,11 +1475,7 @@ int ens_set_variant(se2w_ensemble *e, int variant) {
   for (size_t i = 0; i < e->members.size(); i++) off[i + 1] = off[i] + e->members[i]->ntiles;
   e->total = off.back();
   if (int ierr = upload<int>((void **)&e->d_tile_off, off.data(), off.size(), e->stream)) return ierr;
-  int bpc = 0, thr = 0;
-  ens_occupancy(e->p, e->mode, variant, &bpc, &thr);
-  const long fit = (long)bpc * e->cus;
-  const long want = std::min<long>(fit, ((long)e->total + 7) / 8 * 8);
-  e->grid = (int)std::max<long>(8, want / 8 * 8);
+  e->grid = (int)std::max<long>(8, ((long)e->total + 7) / 8 * 8);
   e->variant = variant;
   ens_drop_graph(e);
   return SE2W_OK;
@@ -1601,14 +1596,12 @@ int se2w_ensemble_create(int32_t n, se2w_ctx *const *members, se2w_ensemble **ou
   e->p = c0->p;
   e->mode = c0->mode;
   int ierr = SE2W_OK;
-  hipDeviceProp_t prop;
-  if (hipGetDeviceProperties(&prop, e->device) != hipSuccess || hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
+  if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess ||
       hipEventCreate(&e->ev2) != hipSuccess) {
     se2w_ensemble_destroy(e);
     return fail(SE2W_ERR_HIP, "stream/event creation failed");
   }
-  e->cus = prop.multiProcessorCount;
   const size_t pbytes = 2 * (size_t)n * (e->mode == SE2W_FP64 ? sizeof(StepParams<double>) : sizeof(StepParams<float>));
   if ((ierr = dev_alloc(&e->d_prms, pbytes))) { se2w_ensemble_destroy(e); return ierr; }
   const int v0 = ens_variant_built(e, c0->variant) ? c0->variant : SE2W_KERNEL_LDS;

@@ -1,7 +1,7 @@
-// se2w_ens.hip -- batched (ensemble) step kernels: a persistent grid walks the
-// tiles of all members; each workgroup runs the single-instance tile bodies
-// with the step parameters of the member that owns the tile.  Compiled once
-// per precision mode with -DSE2W_MODE_TU=<mode>.
+// se2w_ens.hip -- batched (ensemble) step kernels: one workgroup per tile of
+// every member; each workgroup runs the single-instance tile body with the
+// step parameters of the member that owns the tile.  Compiled once per
+// precision mode with -DSE2W_MODE_TU=<mode>.
 #include "se2w_ens.hpp"
 
 #include "se2w.h"
@@ -14,64 +14,60 @@
 
 namespace se2w {
 
-// Global tile t of the ensemble -> (member, member tile), walked by a
-// persistent grid.  The eight XCDs (workgroup b runs on XCD b % 8) each take
-// one contiguous eighth of the tile range, so neighbouring tiles of one
-// member, which re-read each other's halo elements, share an L2.
+// Workgroup b -> global tile t of the ensemble -> (member, member tile).  The
+// grid has one workgroup per tile (8 * ceil(total / 8)); workgroup b runs on
+// XCD b % 8, and the eight XCDs each take one contiguous eighth of the tile
+// range, so neighbouring tiles of one member, which re-read each other's halo
+// elements, share an L2.  One tile per workgroup, with no loop over tiles:
+// the dispatcher starts a workgroup as soon as a slot frees, and no value
+// derived from the thread index is kept live from one tile to the next (a
+// persistent grid walking the tiles measured 10 % slower at p = 6, with 20 %
+// more VGPRs for the same tile body).
 template <typename R, typename Body>
-__device__ __forceinline__ void ens_loop(const EnsArgs<R> &a, Body body) {
-  const int xcd = blockIdx.x & 7, lb = blockIdx.x >> 3, nbx = gridDim.x >> 3;
-  const int per = (a.total + 7) >> 3;
-  const int t0 = xcd * per, t1 = min(t0 + per, a.total);
-  bool first = true;
-  for (int t = t0 + lb; t < t1; t += nbx) {
-    int lo = 0, hi = a.n - 1;  // largest i with tile_off[i] <= t
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (a.tile_off[mid] <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const int i = __builtin_amdgcn_readfirstlane(lo);
-    if (!first) __syncthreads();  // the previous tile's last phase has read its LDS
-    first = false;
-    // The parameter table is read through the constant address space: the
-    // loads are uniform and never clobbered, so they become scalar loads into
-    // SGPRs (a copy through a global pointer would hold the ~60-dword struct in
-    // VGPRs and lower the occupancy).
-    using CWord = const __attribute__((address_space(4))) uint32_t;
-    static_assert(sizeof(StepParams<R>) % 4 == 0, "whole dwords");
-    StepParams<R> prm;
-    {
-      CWord *src = (CWord *)(uintptr_t)(a.prms + 2 * i + (a.step & 1));
-      uint32_t *dst = reinterpret_cast<uint32_t *>(&prm);
-#pragma unroll
-      for (unsigned k = 0; k < sizeof(StepParams<R>) / 4; k++) dst[k] = src[k];
-    }
-    prm.step = a.step;
-    prm.last = a.last;
-    body(prm, t - a.tile_off[i]);
+__device__ __forceinline__ void ens_tile(const EnsArgs<R> &a, Body body) {
+  const int xcd = blockIdx.x & 7, per = (a.total + 7) >> 3;
+  const int t = xcd * per + (blockIdx.x >> 3);
+  if (t >= min((xcd + 1) * per, a.total)) return;  // uniform per workgroup
+  int lo = 0, hi = a.n - 1;  // largest i with tile_off[i] <= t
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (a.tile_off[mid] <= t) lo = mid;
+    else hi = mid - 1;
   }
+  const int i = __builtin_amdgcn_readfirstlane(lo);
+  // The member's parameters are read in place through the constant address
+  // space (the table is written before the run and never by a kernel): each
+  // field is a scalar load where the body uses it, and the pointers loaded
+  // from it address global memory.  A copy of the ~64-dword block into
+  // registers instead held it in SGPRs for the whole tile (spilling them
+  // beside the derivative table) and lost the pointers' address space, so
+  // every state access became a flat load or store.
+  using CP = const __attribute__((address_space(4))) StepParams<R>;
+  CP &prm = *(CP *)(uintptr_t)(a.prms + 2 * i + (a.step & 1));
+  body(prm, t - a.tile_off[i], a.step, a.last);
 }
 
 template <int P, int MODE, int NT>
 __global__ __launch_bounds__(NT) void ens_tile_kernel(EnsArgs<typename RealOf<MODE>::T> a) {
-  ens_loop(a, [](const StepParams<typename RealOf<MODE>::T> &prm, int tile) { tile_body<P, MODE, true, NT>(prm, tile); });
+  ens_tile(a, [](auto &prm, int tile, int step, int last) { tile_body<P, MODE, true, NT>(prm, tile, step, last); });
 }
 
 template <int P, int MODE, int EX, int EY, int WAVES, int PF, int LPE>
 __global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
 void ens_reg_kernel(EnsArgs<float> a) {
-  ens_loop(a, [](const StepParams<float> &prm, int tile) { reg_body<P, MODE, true, EX, EY, PF, LPE>(prm, tile); });
+  ens_tile(a, [](auto &prm, int tile, int step, int last) {
+    reg_body<P, MODE, true, EX, EY, PF, LPE>(prm, tile, step, last);
+  });
 }
 
 template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void ens_mfma_kernel(EnsArgs<float> a) {
-  ens_loop(a, [](const StepParams<float> &prm, int tile) { mfma_body<P, MODE, false>(prm, tile); });
+  ens_tile(a, [](auto &prm, int tile, int step, int last) { mfma_body<P, MODE, false>(prm, tile, step, last); });
 }
 
 template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void ens_line_kernel(EnsArgs<float> a) {
-  ens_loop(a, [](const StepParams<float> &prm, int tile) { mfma_body<P, MODE, true>(prm, tile); });
+  ens_tile(a, [](auto &prm, int tile, int step, int last) { mfma_body<P, MODE, true>(prm, tile, step, last); });
 }
 
 namespace {

@@ -23,7 +23,7 @@ struct EnsArgs {
 };
 
 // Launches the batched step kernel of (p, mode, variant) with `grid`
-// workgroups (a multiple of 8) on stream s.  args points to an
+// workgroups (8 * ceil(tiles / 8): one per tile) on stream s.  args points to an
 // EnsArgs<double> (FP64) or EnsArgs<float>.  Returns 0, or -1 when the
 // variant is not built for (p, mode).
 int ens_launch(int p, int mode, int variant, const void *args, int grid, hipStream_t s);

@@ -111,18 +111,28 @@ __device__ __forceinline__ int tile_of_block(int b, int tiles_per_xcd) {
 // The tile's ranges in the per-tile source, receiver and element-receiver
 // lists, read once per tile through the constant address space (the lists are
 // set up before a run and never written by a kernel): scalar loads, so the
-// per-node update carries no memory access for them.
+// per-node update carries no memory access for them; and the launch's step
+// index and last-step flag.
 struct TileLists {
   int s0, s1;  // sources
   int r0, r1;  // receivers
   int q0, q1;  // element receivers
+  int step;    // step index inside the run
+  int last;    // last step of the run (see StepParams::last)
   __device__ __forceinline__ bool empty() const { return s0 >= s1 && r0 >= r1 && q0 >= q1; }
 };
 
-template <typename R>
-__device__ __forceinline__ TileLists tile_lists(const StepParams<R> &prm, int tile) {
+// The step kernels and their device functions take the step parameters as
+// `PRM &prm`: the kernel argument itself (StepParams<R>) in the single-instance
+// kernels, and in the batched kernels a reference into the members' parameter
+// table in the constant address space, so that every field is a scalar load
+// where it is used and the pointers loaded from it address global memory.
+template <typename PRM>
+__device__ __forceinline__ TileLists tile_lists(PRM &prm, int tile, int step, int last) {
   using CI = const __attribute__((address_space(4))) int;
   TileLists t;
+  t.step = step;
+  t.last = last;
   t.s0 = ((CI *)(uintptr_t)prm.tile_src_ptr)[tile];
   t.s1 = ((CI *)(uintptr_t)prm.tile_src_ptr)[tile + 1];
   t.r0 = ((CI *)(uintptr_t)prm.tile_rec_ptr)[tile];
@@ -276,8 +286,8 @@ __device__ __forceinline__ void force_f32(const float *Sxx, const float *Syy, co
 // vy(i, gj), i = 0..p], i.e. rec_stride(P) = 6 + 2 (P + 1) values.
 __host__ __device__ constexpr int rec_stride(int p) { return 6 + 2 * (p + 1); }
 
-template <int P, typename R>
-__device__ __forceinline__ void record_receivers(const StepParams<R> &prm, const TileLists &tl, int gx, int gy, R ux,
+template <int P, typename R, typename PRM>
+__device__ __forceinline__ void record_receivers(PRM &prm, const TileLists &tl, int gx, int gy, R ux,
                                                  R uy, R vx, R vy, R ax, R ay) {
   constexpr int NB = P + 1;
   for (int t = tl.r0; t < tl.r1; t++) {
@@ -285,7 +295,7 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, const
     const int e = prm.rec_elem[r], g = prm.rec_gll[r];
     const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
     const int gi = g % NB, gj = g / NB;
-    R *o = prm.rec + ((long)prm.step * prm.nr + r) * rec_stride(P);
+    R *o = prm.rec + ((long)tl.step * prm.nr + r) * rec_stride(P);
     if (li == gi && lj >= 0 && lj <= P) o[6 + lj] = vx;
     if (lj == gj && li >= 0 && li <= P) o[6 + NB + li] = vy;
     if (li == gi && lj == gj) {
@@ -298,7 +308,7 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, const
       const int e = prm.erec_elem[r];
       const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
       if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
-        R *o = prm.erec + (((long)prm.step * prm.ner + r) * (NB * NB) + li + lj * NB) * 4;
+        R *o = prm.erec + (((long)tl.step * prm.ner + r) * (NB * NB) + li + lj * NB) * 4;
         o[0] = ux; o[1] = uy; o[2] = vx; o[3] = vy;
       }
     }
@@ -307,15 +317,15 @@ __device__ __forceinline__ void record_receivers(const StepParams<R> &prm, const
 
 // FP64: a = f / M; v += (0.5 dt) a; record; u += dt v; u += (0.5 dt^2) a;
 // v += (0.5 dt) a -- each operation rounded as in the reference's AXPYs.
-template <int P>
-__device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, const TileLists &tl, int gx, int gy, long idx,
+template <int P, typename PRM>
+__device__ __forceinline__ void node_finish_f64(PRM &prm, const TileLists &tl, int gx, int gy, long idx,
                                                 double fx, double fy, double ux, double uy, double2 vv, double mass) {
 #pragma clang fp contract(off)
   const double ax = fx / mass, ay = fy / mass;
   const double vx = vv.x + prm.hdt * ax, vy = vv.y + prm.hdt * ay;
   record_receivers<P, double>(prm, tl, gx, gy, ux, uy, vx, vy, ax, ay);
   double2 *vdst = prm.v;
-  if (prm.last) {
+  if (tl.last) {
     prm.v[idx] = double2{vx, vy};
     prm.a[idx] = double2{ax, ay};
     if (!prm.v_next) {
@@ -333,8 +343,8 @@ __device__ __forceinline__ void node_finish_f64(const StepParams<double> &prm, c
 
 // FP32: same sequence with fused multiply-adds; MODE_F32K accumulates the
 // displacement with a compensated (Kahan) sum, one per reference AXPY.
-template <int P, int MODE>
-__device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, const TileLists &tl, int gx, int gy, long idx,
+template <int P, int MODE, typename PRM>
+__device__ __forceinline__ void node_finish_f32(PRM &prm, const TileLists &tl, int gx, int gy, long idx,
                                                 float fx, float fy, float ux, float uy, float2 vv, float minv,
                                                 float2 cc) {
   const float ax = fx * minv, ay = fy * minv;
@@ -342,7 +352,7 @@ __device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, co
   record_receivers<P, float>(prm, tl, gx, gy, ux, uy, vx, vy, ax, ay);
   float2 *vdst = prm.v;
   [[maybe_unused]] float2 *cdst = prm.c_out;
-  if (prm.last) {
+  if (tl.last) {
     prm.v[idx] = float2{vx, vy};
     prm.a[idx] = float2{ax, ay};
     if (!prm.v_next) {
@@ -374,8 +384,8 @@ __device__ __forceinline__ void node_finish_f32(const StepParams<float> &prm, co
 // ---------------------------------------------------------------------------
 // Node update with the nodal v, mass and compensation already loaded.
 // ---------------------------------------------------------------------------
-template <int P, int MODE>
-__device__ __forceinline__ void node_update_core(const StepParams<typename RealOf<MODE>::T> &prm, const TileLists &tl,
+template <int P, int MODE, typename PRM>
+__device__ __forceinline__ void node_update_core(PRM &prm, const TileLists &tl,
                                                  int gx, int gy, long idx, typename RealOf<MODE>::T fx,
                                                  typename RealOf<MODE>::T fy, typename RealOf<MODE>::T ux,
                                                  typename RealOf<MODE>::T uy, typename V2<typename RealOf<MODE>::T>::T vv,
@@ -395,7 +405,7 @@ __device__ __forceinline__ void node_update_core(const StepParams<typename RealO
         const int li = gx - (e % prm.mx) * P, lj = gy - (e / prm.mx) * P;
         if (li >= 0 && li <= P && lj >= 0 && lj <= P) {
           const int k = li + lj * NB;
-          const R ps = prm.psi[(long)prm.step * prm.ns + s];
+          const R ps = prm.psi[(long)tl.step * prm.ns + s];
           gxv = gxv + ps * prm.src_vals[(long)s * 2 * NB * NB + 2 * k];
           gyv = gyv + ps * prm.src_vals[(long)s * 2 * NB * NB + 2 * k + 1];
           hit = true;
@@ -418,8 +428,8 @@ __device__ __forceinline__ void node_update_core(const StepParams<typename RealO
 // Sequence per node (se2wave.c:4739-4745 then 4726-4730):
 //   f = f + g;  a = f / M;  v = v + (0.5 dt) a;  [record u, v, a]
 //   u = u + dt v;  u = u + (0.5 dt^2) a;  v = v + (0.5 dt) a.
-template <int P, int MODE>
-__device__ __forceinline__ void node_update(const StepParams<typename RealOf<MODE>::T> &prm, const TileLists &tl,
+template <int P, int MODE, typename PRM>
+__device__ __forceinline__ void node_update(PRM &prm, const TileLists &tl,
                                             int gx, int gy, typename RealOf<MODE>::T fx, typename RealOf<MODE>::T fy,
                                             typename RealOf<MODE>::T ux, typename RealOf<MODE>::T uy) {
   using R = typename RealOf<MODE>::T;
@@ -474,8 +484,8 @@ __device__ __forceinline__ void owned_node(int k, int OX, int OY, int &ox, int &
 // A workgroup that calls it for several tiles in a row must put a barrier
 // between the calls: the last phase of one call reads `su`, which the first
 // phase of the next call overwrites.
-template <int P, int MODE, bool FUSED, int NT = kThreads>
-__device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>::T> &prm, int tile) {
+template <int P, int MODE, bool FUSED, int NT = kThreads, typename PRM>
+__device__ __forceinline__ void tile_body(PRM &prm, int tile, int step, int last) {
   using R = typename RealOf<MODE>::T;
   using R2 = typename V2<R>::T;
   constexpr int NB = P + 1;
@@ -608,7 +618,7 @@ __device__ __forceinline__ void tile_body(const StepParams<typename RealOf<MODE>
   }
   __syncthreads();
   // Phase C (+D): owned nodes.
-  const TileLists tl = tile_lists(prm, tile);
+  const TileLists tl = tile_lists(prm, tile, step, last);
 #pragma unroll
   for (int sl = 0; sl < NSLOT; sl++) {
     if (tid + sl * NT >= OX * OY) break;
@@ -668,7 +678,7 @@ template <int P, int MODE, bool FUSED, int NT = kThreads>
 __global__ __launch_bounds__(NT) void tile_kernel(StepParams<typename RealOf<MODE>::T> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  tile_body<P, MODE, FUSED, NT>(prm, tile);
+  tile_body<P, MODE, FUSED, NT>(prm, tile, prm.step, prm.last);
 }
 
 // Update kernel of the split schedule: one workgroup per tile of owned nodes.
@@ -685,7 +695,7 @@ __global__ __launch_bounds__(kThreads) void update_kernel(StepParams<typename Re
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
-  const TileLists tl = tile_lists(prm, tile);
+  const TileLists tl = tile_lists(prm, tile, prm.step, prm.last);
   for (int k = threadIdx.x; k < OX * OY; k += kThreads) {
     const int oy = k / OX, ox = k - oy * OX;
     const int gx = ex0 * P + ox, gy = ey0 * P + oy;

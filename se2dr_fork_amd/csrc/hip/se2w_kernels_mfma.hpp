@@ -104,8 +104,8 @@ __device__ __forceinline__ void line_phase(const float *__restrict__ D, Load loa
   }
 }
 
-template <int P, int MODE, bool LINES>
-__device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile) {
+template <int P, int MODE, bool LINES, typename PRM>
+__device__ __forceinline__ void mfma_body(PRM &prm, int tile, int step, int last) {
   // LINES = false: contractions on the matrix cores; true: on the vector ALUs,
   // one line per thread (same data layout and phases).
   static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
@@ -309,7 +309,7 @@ __device__ __forceinline__ void mfma_body(const StepParams<float> &prm, int tile
   }
   __syncthreads();
   // D: owned nodes: element contributions in ascending element order, update.
-  const TileLists tl = tile_lists(prm, tile);
+  const TileLists tl = tile_lists(prm, tile, step, last);
   const int ex_end = min(ex0 + TEX, prm.mx), ey_end = min(ey0 + TEY, prm.my);
   const int OX = (ex_end - ex0) * P + (ex_end == prm.mx ? 1 : 0);
   const int OY = (ey_end - ey0) * P + (ey_end == prm.my ? 1 : 0);
@@ -352,14 +352,14 @@ template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void mfma_kernel(StepParams<float> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  mfma_body<P, MODE, false>(prm, tile);
+  mfma_body<P, MODE, false>(prm, tile, prm.step, prm.last);
 }
 
 template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void line_kernel(StepParams<float> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  mfma_body<P, MODE, true>(prm, tile);
+  mfma_body<P, MODE, true>(prm, tile, prm.step, prm.last);
 }
 
 }  // namespace se2w

@@ -62,8 +62,8 @@ __device__ __forceinline__ float &comp(float2 &v, int c) { return reinterpret_ca
 // One workgroup of EX x EY x LPE threads processes one tile of (EX-1) x
 // (EY-1) elements (all threads call it with the same tile).  A workgroup that
 // calls it for several tiles in a row puts a barrier between the calls.
-template <int P, int MODE, bool FUSED, int EX, int EY, int PF, int LPE>
-__device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile) {
+template <int P, int MODE, bool FUSED, int EX, int EY, int PF, int LPE, typename PRM>
+__device__ __forceinline__ void reg_body(PRM &prm, int tile, int step, int last) {
   static_assert(MODE == MODE_F32 || MODE == MODE_F32K, "FP32 kernel");
   static_assert(LPE == 1 || LPE == 2, "one or two lanes per element");
   constexpr int NB = P + 1;
@@ -305,7 +305,7 @@ __device__ __forceinline__ void reg_body(const StepParams<float> &prm, int tile)
       for (int cidx = 0; cidx < CPL; cidx++) node_loads(r, cidx, true, PF == 2);
   }
   __syncthreads();
-  const TileLists tl = tile_lists(prm, tile);
+  const TileLists tl = tile_lists(prm, tile, step, last);
   constexpr int NSL = RPW * CPL, CHUNK = 8;
   constexpr bool CHUNKED = PF == 4;
   if constexpr (FUSED && PF < 2 && !CHUNKED) {
@@ -393,7 +393,7 @@ __global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(W
 void reg_kernel(StepParams<float> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  reg_body<P, MODE, FUSED, EX, EY, PF, LPE>(prm, tile);
+  reg_body<P, MODE, FUSED, EX, EY, PF, LPE>(prm, tile, prm.step, prm.last);
 }
 
 }  // namespace se2w

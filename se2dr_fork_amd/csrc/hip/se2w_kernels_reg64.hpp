@@ -65,8 +65,8 @@ __device__ __forceinline__ double &comp64(double2 &v, int c) { return reinterpre
 // One workgroup of EX x EY x 2 threads processes one tile of (EX-1) x (EY-1)
 // elements (all threads call it with the same tile).  A workgroup that calls
 // it for several tiles in a row puts a barrier between the calls.
-template <int P, int EX, int EY, int PF>
-__device__ __forceinline__ void reg64_body(const StepParams<double> &prm, int tile) {
+template <int P, int EX, int EY, int PF, typename PRM>
+__device__ __forceinline__ void reg64_body(PRM &prm, int tile, int step, int last) {
 #pragma clang fp contract(off)
   constexpr int NB = P + 1;
   constexpr int TEX = EX - 1, TEY = EY - 1;
@@ -248,7 +248,7 @@ __device__ __forceinline__ void reg64_body(const StepParams<double> &prm, int ti
     pm[r][cidx] = prm.mass[idx];
   };
   __syncthreads();
-  const TileLists tl = tile_lists(prm, tile);
+  const TileLists tl = tile_lists(prm, tile, step, last);
   constexpr int NSL = RPW * CPL, CHUNK = 8;
   if constexpr (PF == 0) {
 #pragma unroll
@@ -314,8 +314,8 @@ __device__ __forceinline__ void reg64_body(const StepParams<double> &prm, int ti
 // term of the reference's loop itself (no cross-lane moves), in the same
 // order as the lane-pair body above.  A workgroup of EX x EY threads
 // processes one tile of (EX-1) x (EY-1) elements.
-template <int P, int EX, int EY, int PF>
-__device__ __forceinline__ void reg64s_body(const StepParams<double> &prm, int tile) {
+template <int P, int EX, int EY, int PF, typename PRM>
+__device__ __forceinline__ void reg64s_body(PRM &prm, int tile, int step, int last) {
 #pragma clang fp contract(off)
   constexpr int NB = P + 1;
   constexpr int TEX = EX - 1, TEY = EY - 1;
@@ -483,7 +483,7 @@ __device__ __forceinline__ void reg64s_body(const StepParams<double> &prm, int t
     pm[r][cidx] = prm.mass[idx];
   };
   __syncthreads();
-  const TileLists tl = tile_lists(prm, tile);
+  const TileLists tl = tile_lists(prm, tile, step, last);
   constexpr int NSL = RPW * CPL, CHUNK = 8;
   if constexpr (PF == 0) {
 #pragma unroll
@@ -551,8 +551,8 @@ __global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(W
 void reg64_kernel(StepParams<double> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
-  if constexpr (LPE == 2) reg64_body<P, EX, EY, PF>(prm, tile);
-  else reg64s_body<P, EX, EY, PF>(prm, tile);
+  if constexpr (LPE == 2) reg64_body<P, EX, EY, PF>(prm, tile, prm.step, prm.last);
+  else reg64s_body<P, EX, EY, PF>(prm, tile, prm.step, prm.last);
 }
 
 }  // namespace se2w
