@@ -110,7 +110,7 @@ def measured_traffic(mx, p, prec, signature):
 # (elements in x, elements in y, minimum waves per SIMD, node-phase prefetch
 # level, lanes per element) of the register-blocked kernel's schedules, in the
 # library's variant order (SE2W_KERNEL_REG + k)
-REG_SCHEDULES = [(32, 4, 0, 2, 1), (16, 4, 0, 2, 1), (32, 2, 0, 2, 1), (32, 4, 0, 0, 2), (16, 4, 0, 4, 2),
+REG_SCHEDULES = [(32, 4, 0, 4, 1), (64, 2, 0, 4, 1), (32, 2, 0, 2, 1), (32, 4, 0, 0, 2), (16, 4, 0, 4, 2),
                  (16, 8, 0, 4, 2), (16, 4, 0, 2, 2), (16, 4, 0, 0, 2)]
 # (elements in x, elements in y, minimum waves per SIMD, node-load placement,
 # lanes per element) of the FP64 register kernel's schedules

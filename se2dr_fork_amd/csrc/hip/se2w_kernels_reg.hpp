@@ -36,7 +36,7 @@ namespace se2w {
 // reg_body) and the lanes per element.  The autotuner picks one per
 // problem.
 struct RegSchedule { int ex, ey, waves, pf, lpe; };
-constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 2, 1}, {16, 4, 0, 2, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
+constexpr RegSchedule kRegSchedules[] = {{32, 4, 0, 4, 1}, {64, 2, 0, 4, 1}, {32, 2, 0, 2, 1}, {32, 4, 0, 0, 2},
                                          {16, 4, 0, 4, 2}, {16, 8, 0, 4, 2}, {16, 4, 0, 2, 2}, {16, 4, 0, 0, 2}};
 constexpr int kNumRegSchedules = sizeof(kRegSchedules) / sizeof(kRegSchedules[0]);
 
