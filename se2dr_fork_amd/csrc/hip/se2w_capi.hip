@@ -187,6 +187,7 @@ struct se2w_ctx {
   int *d_src_elem = nullptr, *d_tsrc_ptr = nullptr, *d_tsrc_idx = nullptr;
   void *d_src_vals = nullptr;
   void *d_psi = nullptr;
+  void *h_psi = nullptr;  // pinned host copy of a run's source-time table (psi_cap values)
   size_t psi_cap = 0;
   // receivers
   int nr = 0;
@@ -196,6 +197,7 @@ struct se2w_ctx {
   size_t rec_cap = 0;
   void *h_stage = nullptr;  // pinned host staging buffer for the record download
   size_t stage_cap = 0;
+  bool rec_staged = false;  // h_stage holds the last run's receiver record
   // element receivers (u, v at every node of an element, for interpolation)
   int ner = 0;
   std::vector<int> h_erec_elem;
@@ -420,6 +422,19 @@ int ensure_stage_for(se2w_ctx *c, int nsteps) {
 
 int ensure_stage(se2w_ctx *c) { return ensure_stage_for(c, c->rec_steps); }
 
+// Enqueues on stream s, behind a run's kernels, the download of its receiver
+// record into the pinned staging buffer (grown while the kernels execute),
+// so that the wait at the run's end covers it and se2w_get_receivers only
+// converts host memory.
+int stage_record(se2w_ctx *c, hipStream_t s) {
+  if (c->nr == 0 || c->rec_steps == 0) return SE2W_OK;
+  if (int ierr = ensure_stage(c)) return ierr;
+  const size_t bytes = (size_t)c->rec_steps * c->nr * rec_stride(c->p) * real_size(c->mode);
+  HIPCHK(hipMemcpyAsync(c->h_stage, c->d_rec, bytes, hipMemcpyDeviceToHost, s));
+  c->rec_staged = true;
+  return SE2W_OK;
+}
+
 // Device buffers of a run of nsteps steps: the source-time table, the
 // receiver record and the element-receiver record, grown (never shrunk) to
 // the run's length.
@@ -430,8 +445,11 @@ int ensure_run_buffers(se2w_ctx *c, int nsteps) {
     const size_t n = (size_t)nsteps * c->ns;
     if (n > c->psi_cap) {
       dev_free(c->d_psi);
+      if (c->h_psi) HIPCHK(hipHostFree(c->h_psi));
+      c->h_psi = nullptr;
       c->psi_cap = 0;
       if ((ierr = dev_alloc(&c->d_psi, n * rs))) return ierr;
+      HIPCHK(hipHostMalloc(&c->h_psi, n * rs, hipHostMallocDefault));
       c->psi_cap = n;
     }
   }
@@ -522,8 +540,8 @@ int run_mode(se2w_ctx *c, int nsteps, double dt) {
   HIPCHK(hipGetLastError());
   close_run(c, dt);
   HIPCHK(hipEventRecord(c->ev1, c->stream));
-  if (int ierr = ensure_stage(c)) return ierr;
-  HIPCHK(hipEventSynchronize(c->ev1));
+  if (int ierr = stage_record(c, c->stream)) return ierr;
+  HIPCHK(hipStreamSynchronize(c->stream));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, c->ev0, c->ev1));
   c->last_ms = ms;
@@ -694,6 +712,7 @@ int se2w_destroy(se2w_ctx *c) {
   if (c->snap_done) hipEventDestroy(c->snap_done);
   if (c->copy_stream) hipStreamDestroy(c->copy_stream);
   if (c->h_stage) hipHostFree(c->h_stage);
+  if (c->h_psi) hipHostFree(c->h_psi);
   if (c->d_partial) hipFree(c->d_partial);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -963,7 +982,9 @@ namespace {
 
 // Checks a run's arguments and stages its inputs: the source-time table and
 // the receiver record buffer.
-int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
+// Checks a run's arguments, sizes its buffers and enqueues its source-time
+// table on stream s, the stream its kernels will run on.
+int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi, hipStream_t s) {
   if (!c) return fail(SE2W_ERR_ARG, "null context");
   if (nsteps < 1) return fail(SE2W_ERR_ARG, "nsteps must be >= 1");
   if (!(dt > 0.0) || !std::isfinite(dt)) return fail(SE2W_ERR_ARG, "dt must be positive and finite");
@@ -972,19 +993,20 @@ int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
     return fail(SE2W_ERR_STATE, "set tables, cells and mass before se2w_run");
   HIPCHK(hipSetDevice(c->device));
   if (int ierr = ensure_run_buffers(c, nsteps)) return ierr;
-  // source-time table for this run
+  // Source-time table for this run, through the pinned buffer: the copy is
+  // ordered before the run's kernels on the stream, and the buffer is free
+  // again when the previous run (which synchronised at its end) is done.
   if (c->ns > 0) {
     const size_t n = (size_t)nsteps * c->ns;
     if (c->mode == SE2W_FP64) {
-      HIPCHK(hipMemcpyAsync(c->d_psi, psi, n * sizeof(double), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+      memcpy(c->h_psi, psi, n * sizeof(double));
     } else {
-      std::vector<float> tmp(n);
-      for (size_t i = 0; i < n; i++) tmp[i] = (float)psi[i];
-      HIPCHK(hipMemcpyAsync(c->d_psi, tmp.data(), n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(hipStreamSynchronize(c->stream));
+      float *h = (float *)c->h_psi;
+      for (size_t i = 0; i < n; i++) h[i] = (float)psi[i];
     }
+    HIPCHK(hipMemcpyAsync(c->d_psi, c->h_psi, n * real_size(c->mode), hipMemcpyHostToDevice, s));
   }
+  c->rec_staged = false;
   c->rec_steps = nsteps;
   c->erec_steps = nsteps;
   c->trace.clear();
@@ -1037,7 +1059,7 @@ int se2w_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi) {
   // the run's first and last events (stderr).
   static const bool trace = getenv("SE2W_PHASE_TIMES") != nullptr;
   const auto t0 = std::chrono::steady_clock::now();
-  int ierr = prepare_run(c, nsteps, dt, psi);
+  int ierr = prepare_run(c, nsteps, dt, psi, c->stream);
   if (ierr) return ierr;
   const auto t1 = std::chrono::steady_clock::now();
   if (c->mode == SE2W_FP64) ierr = run_mode<MODE_F64>(c, nsteps, dt);
@@ -1061,9 +1083,11 @@ int se2w_get_receivers(se2w_ctx *c, double *trace) {
   const int nb = c->nb, nr = c->nr, rs = rec_stride(c->p);
   const size_t n = (size_t)c->rec_steps * nr * rs;
   const size_t bytes = n * real_size(c->mode);
-  if (int ierr = ensure_stage(c)) return ierr;
-  HIPCHK(hipMemcpyAsync(c->h_stage, c->d_rec, bytes, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(hipStreamSynchronize(c->stream));
+  if (!c->rec_staged) {
+    if (int ierr = ensure_stage(c)) return ierr;
+    HIPCHK(hipMemcpyAsync(c->h_stage, c->d_rec, bytes, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   const double *rd = (const double *)c->h_stage;
   const float *rf = (const float *)c->h_stage;
   const bool f64 = c->mode == SE2W_FP64;
@@ -1547,9 +1571,9 @@ int ens_run_mode(se2w_ensemble *e, int nsteps, const double *dt) {
     se2w_ctx *c = e->members[i];
     c->cur ^= ((nsteps - 1) & 1);  // the last step's input: the plain displacement
     close_run(c, dt[i]);
-    if (int ierr = ensure_stage(c)) return ierr;
+    if (int ierr = stage_record(c, e->stream)) return ierr;
   }
-  HIPCHK(hipEventSynchronize(e->ev2));
+  HIPCHK(hipStreamSynchronize(e->stream));
   float a = 0.f, b = 0.f;
   HIPCHK(hipEventElapsedTime(&a, e->ev0, e->ev2));
   HIPCHK(hipEventElapsedTime(&b, e->ev1, e->ev2));
@@ -1562,7 +1586,7 @@ int ens_run(se2w_ensemble *e, int nsteps, const double *dt, const double *const 
   HIPCHK(hipSetDevice(e->device));
   for (size_t i = 0; i < e->members.size(); i++) {
     se2w_ctx *c = e->members[i];
-    if (int ierr = prepare_run(c, nsteps, dt[i], psi ? psi[i] : nullptr)) return ierr;
+    if (int ierr = prepare_run(c, nsteps, dt[i], psi ? psi[i] : nullptr, e->stream)) return ierr;
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   if (e->mode == SE2W_FP64) return ens_run_mode<MODE_F64>(e, nsteps, dt);
