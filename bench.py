@@ -473,6 +473,7 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
         ref_state = b.solver.ctx.get_state()[:2]
         out["simulated_s"] = full_steps * b.dt
         b.solver.close()
+        log(f"variants: FP64 reference run of {full_steps} steps in {out['fp64_s']:.1f} s")
     for name in precisions:
         prec = S.PRECISIONS[name]
         for v in range(hip.se2w_kernel_count()):
@@ -498,6 +499,7 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
                 row["state_rel_l2_vs_fp64"] = {"u": rel_l2(u, ref_state[0]), "v": rel_l2(v, ref_state[1])}
                 del u, v
             out["variants"].append(row)
+            log(f"variants: {name} variant {v} ({row['kernel']}) {row['kernel_ms']:.4f} ms")
             b.solver.close()
     return out
 

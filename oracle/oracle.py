@@ -32,6 +32,7 @@ class OrcScenario(C.Structure):
         ("src_impl", C.POINTER(C.c_int)),
         ("nuv", C.c_int), ("uv_xy", C.POINTER(C.c_double)), ("uv_interp", C.POINTER(C.c_int)),
         ("rec_dir", C.c_char_p),
+        ("src_h", C.POINTER(C.c_double)),
     ]
 
 
@@ -107,6 +108,8 @@ class Problem:
         s.uv_interp = _ip(keep(np.asarray(uv_interp if uv_interp is not None else [0] * nuv), np.int32)) if nuv else None
         self._rec_dir = None if rec_dir is None else str(rec_dir).encode()
         s.rec_dir = self._rec_dir
+        h = getattr(sc, "src_h", None)
+        s.src_h = _dp(keep(np.asarray(h, dtype=np.float64))) if (h is not None and ns) else None
         self.nuv = nuv
         self.s = s
         self.sc = sc
@@ -140,6 +143,14 @@ class Problem:
         vals = np.zeros(2 * self.npe)
         _chk(lib().orc_source_values(C.byref(self.s), s, C.byref(e), C.byref(g), _dp(vals)))
         return e.value, g.value, vals
+
+    def assembled_source(self, s):
+        """(F[2 nnodes], kernel width h, \\int W) of source s: the assembled
+        nodal vector (h and \\int W are those of a cubic-spline source, else 0)."""
+        F = np.zeros(2 * self.nnodes)
+        h, iw = C.c_double(), C.c_double()
+        _chk(lib().orc_source_assembled(C.byref(self.s), s, _dp(F), C.byref(h), C.byref(iw)))
+        return F, h.value, iw.value
 
     def receivers(self):
         nr = len(self.sc.rec_xy)

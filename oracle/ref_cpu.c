@@ -387,13 +387,17 @@ typedef struct {
   int nthreads;
   /* Optional fields (zero / NULL keeps the behaviour above). */
   const int *src_impl;  /* [nsrc]: 1 = nearest GLL node (SOURCE_IMPL_NEAREST_QPOINT, also when NULL),
-                           0 = basis at the source's own local coordinate (SOURCE_IMPL_POINTWISE, EXT) */
+                           0 = basis at the source's own local coordinate (SOURCE_IMPL_POINTWISE, EXT),
+                           2 = cubic-spline kernel (the legacy cspline source, 1925-2057),
+                           4 = P0 cell source (the legacy P0 source, 2059-2198); 2 and 4: moment only */
   int nuv;              /* single-station receivers, RecordUV (se2wave.c:2351-2443) and
                            RecordUV_interp (2445-2581) */
   const double *uv_xy;  /* [nuv][2] */
   const int *uv_interp; /* [nuv]: 0 = nearest node of the element, 1 = Lagrange interpolation */
   const char *rec_dir;  /* non-NULL: every step appends its row to the receiver file in this
                            directory (fopen/fprintf/fclose per step, 2904-2981) */
+  const double *src_h;  /* [nsrc] or NULL: kernel width h of a cubic-spline source; <= 0 takes the
+                           reference's default (1959-1973), > 0 replaces it as -sm_h does (1974-1977) */
 } orc_scenario;
 
 typedef struct {
@@ -691,7 +695,109 @@ static double orc_timestep(const orc_mesh *m)
 typedef struct {
   int elem, closest_gll;
   double vals[2 * 11 * 11]; /* 2*npe, p <= 10 */
+  double *F;                /* cubic-spline source: the assembled nodal vector [2N] (elem = -1) */
+  double kernel_h, int_w;   /* cubic-spline source: kernel width and the printed \int W */
 } orc_source;
+
+/* The cubic-spline smoothing kernel of AssembleLinearForm_ElastoDynamicsMomentDirac2d_Kernel_CSpline
+ * (se2wave.c:2011-2022), W(r) with support 2h, normalised to 1 in 2D. */
+static double orc_spline_W(double sep, double h)
+{
+  double w, arg = sqrt(sep) / h;
+  if (arg < 1.0) w = 0.25 * (2.0 - arg) * (2.0 - arg) * (2.0 - arg) - (1.0 - arg) * (1.0 - arg) * (1.0 - arg);
+  else if (arg < 2.0) w = 0.25 * (2.0 - arg) * (2.0 - arg) * (2.0 - arg);
+  else w = 0.0;
+  return 10.0 / (7.0 * ORC_PI) * w * (1.0 / (h * h));
+}
+
+/* AssembleLinearForm_ElastoDynamicsMomentDirac2d_Kernel_CSpline, se2wave.c:1925-2057, one moment
+ * source: the kernel width from the bounding box (1940-1973, or h_opt > 0 as -sm_h), then for every
+ * element in ascending order fe_i = sum_q (w_q detJ) (M . grad N_i(x_q)) W(|x_q - x_s|) over the
+ * GLL points, assembled into F in element order (VecSetValues ADD_VALUES). */
+static int orc_source_spline(const orc_mesh *m, const double coor[2], const double *M, double h_opt, orc_source *s)
+{
+  int npe = m->npe, e, q, i;
+  double dx = (m->gmax[0] - m->gmin[0]) / ((double)m->mx), dy = (m->gmax[1] - m->gmin[1]) / ((double)m->my);
+  double ds = dx, h, elc[2 * 121], fe[2 * 121], dN_dx[121 * 121], dN_dy[121 * 121];
+  if (dy > dx) ds = dy;
+  if (m->p <= 3) h = 1.5 * (0.5 * ds);
+  else {
+    double pfac = 3.0 / (m->p - 2.0);
+    h = pfac * (0.5 * ds);
+  }
+  if (h_opt > 0.0) h = h_opt;
+  s->elem = -1;
+  s->closest_gll = -1;
+  s->kernel_h = h;
+  s->int_w = 0.0;
+  s->F = (double *)calloc((size_t)2 * m->N, sizeof(double));
+  if (!s->F) return orc_fail(ORC_ERR_MEM, "out of memory");
+  for (e = 0; e < m->ne; e++) {
+    const int *elnidx = &m->element[(size_t)npe * e];
+    double detJ;
+    for (i = 0; i < npe; i++) {
+      elc[2 * i] = m->coor[2 * (long)elnidx[i]];
+      elc[2 * i + 1] = m->coor[2 * (long)elnidx[i] + 1];
+    }
+    for (i = 0; i < 2 * npe; i++) fe[i] = 0.0;
+    orc_derivs(m, npe, elc, m->dN_dxi, m->dN_deta, dN_dx, dN_dy);
+    detJ = orc_detJ(m, elc);
+    for (q = 0; q < npe; q++) {
+      const double *x_q = &elc[2 * q];
+      double sep = (x_q[0] - coor[0]) * (x_q[0] - coor[0]);
+      double W;
+      sep += (x_q[1] - coor[1]) * (x_q[1] - coor[1]);
+      W = orc_spline_W(sep, h);
+      for (i = 0; i < npe; i++) {
+        fe[2 * i] += m->w[q] * detJ * (M[0] * dN_dx[q * npe + i] + M[1] * dN_dy[q * npe + i]) * W;
+        fe[2 * i + 1] += m->w[q] * detJ * (M[2] * dN_dx[q * npe + i] + M[3] * dN_dy[q * npe + i]) * W;
+      }
+      s->int_w += m->w[q] * detJ * W;
+    }
+    for (i = 0; i < npe; i++) {
+      s->F[2 * (long)elnidx[i]] += fe[2 * i];
+      s->F[2 * (long)elnidx[i] + 1] += fe[2 * i + 1];
+    }
+  }
+  return ORC_OK;
+}
+
+/* AssembleLinearForm_ElastoDynamicsMomentDirac2d_P0, se2wave.c:2059-2198, one moment source: the
+ * cell containing it (from the bounding box, 2089-2108), the physical basis derivatives at the cell
+ * centre xi = (0, 0) (2133-2148), and fe_i = sum_q (w_q detJ) (M . grad N_i(centre)) / (dx dy). */
+static int orc_source_p0(const orc_mesh *m, const double coor[2], const double *M, orc_source *s)
+{
+  int npe = m->npe, ii, jj, e, q, i;
+  double dx = (m->gmax[0] - m->gmin[0]) / ((double)m->mx), dy = (m->gmax[1] - m->gmin[1]) / ((double)m->my);
+  double xi[2] = {0.0, 0.0}, elc[2 * 121], dN_dxi_q[121], dN_deta_q[121], N_q[121], dNx[121], dNy[121], detJ;
+  ii = (int)((coor[0] - m->gmin[0]) / dx);
+  jj = (int)((coor[1] - m->gmin[1]) / dy);
+  if (ii == m->mx) ii--;
+  if (jj == m->my) jj--;
+  if (ii < 0) return orc_fail(ORC_ERR_USER, "source: x < gmin[0]");
+  if (jj < 0) return orc_fail(ORC_ERR_USER, "source: y < gmin[1]");
+  if (ii > m->mx) return orc_fail(ORC_ERR_USER, "source: x > gmax[0]");
+  if (jj > m->my) return orc_fail(ORC_ERR_USER, "source: y > gmax[1]");
+  e = ii + jj * m->mx;
+  for (i = 0; i < npe; i++) {
+    long nidx = m->element[(size_t)npe * e + i];
+    elc[2 * i] = m->coor[2 * nidx];
+    elc[2 * i + 1] = m->coor[2 * nidx + 1];
+  }
+  orc_tensor2d_at_point(xi, m->p, dN_dxi_q, dN_deta_q, N_q);
+  orc_derivs(m, 1, elc, dN_dxi_q, dN_deta_q, dNx, dNy);
+  detJ = orc_detJ(m, elc);
+  s->elem = e;
+  s->closest_gll = -1;
+  for (i = 0; i < 2 * npe; i++) s->vals[i] = 0.0;
+  for (q = 0; q < npe; q++) {
+    for (i = 0; i < npe; i++) {
+      s->vals[2 * i] += m->w[q] * detJ * (M[0] * dNx[i] + M[1] * dNy[i]) / (dx * dy);
+      s->vals[2 * i + 1] += m->w[q] * detJ * (M[2] * dNx[i] + M[3] * dNy[i]) / (dx * dy);
+    }
+  }
+  return ORC_OK;
+}
 
 /* SeismicSourceCreate 2995-3243 (single rank) + SeismicSourceSetup_Pointwise
  * 3310-3365 + SeismicSourceSetup_Moment_Pointwise 3367-3429.  EXT: force;
@@ -700,12 +806,19 @@ typedef struct {
  * (3097-3104) as SeismicSourceSetup_Pointwise's pointwise branch does
  * (3339-3342), with no GLL node search. */
 static int orc_source_setup(const orc_mesh *m, const double coor[2], int type, const double *values, int use_internal,
-                            int impl, orc_source *s)
+                            int impl, double h_opt, orc_source *s)
 {
   double dx, dy, gmin_d[2], gmax_d[2], xi_src[2];
   int ii, jj, e, i, nj, ni, min_qp = -1, npe = m->npe;
   double elc[2 * 121] = {0};
   if (m->p > 10) return orc_fail(ORC_ERR_SUP, "p > 10");
+  s->F = NULL;
+  if (impl == 2 || impl == 4) {
+    if (type != 1) return orc_fail(ORC_ERR_SUP, "the cubic-spline and P0 sources are moment sources");
+    if (coor[0] < m->gmin[0] || coor[1] < m->gmin[1] || coor[0] > m->gmax[0] || coor[1] > m->gmax[1])
+      return orc_fail(ORC_ERR_USER, "source outside the domain");
+    return impl == 2 ? orc_source_spline(m, coor, values, h_opt, s) : orc_source_p0(m, coor, values, s);
+  }
   dx = (m->gmax[0] - m->gmin[0]) / ((double)m->mx);
   dy = (m->gmax[1] - m->gmin[1]) / ((double)m->my);
   if (coor[0] < m->gmin[0] || coor[1] < m->gmin[1] || coor[0] > m->gmax[0] || coor[1] > m->gmax[1])
@@ -952,7 +1065,10 @@ int orc_compute_timestep(const orc_scenario *sc, double *dt)
 }
 
 static int orc_src_impl(const orc_scenario *sc, int s) { return sc->src_impl ? sc->src_impl[s] : 1; }
+static double orc_src_h(const orc_scenario *sc, int s) { return sc->src_h ? sc->src_h[s] : 0.0; }
 
+/* Element, closest GLL node and nodal values of source s (a cubic-spline
+ * source has no single element: elem = -1, see orc_source_assembled). */
 int orc_source_values(const orc_scenario *sc, int s, int *elem, int *closest_gll, double *vals)
 {
   orc_mesh m;
@@ -960,12 +1076,46 @@ int orc_source_values(const orc_scenario *sc, int s, int *elem, int *closest_gll
   int ierr = orc_mesh_create(sc, &m);
   if (ierr) return ierr;
   ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll,
-                          orc_src_impl(sc, s), &src);
+                          orc_src_impl(sc, s), orc_src_h(sc, s), &src);
   if (!ierr) {
     *elem = src.elem;
     *closest_gll = src.closest_gll;
     memcpy(vals, src.vals, sizeof(double) * 2 * m.npe);
   }
+  free(src.F);
+  orc_mesh_free(&m);
+  return ierr;
+}
+
+/* The assembled nodal vector F[2N] of source s (any implementation: an
+ * element source's values placed at its element's nodes), and for a
+ * cubic-spline source its kernel width h and \int W (the lines
+ * se2wave.c:1971 and 2054 print). */
+int orc_source_assembled(const orc_scenario *sc, int s, double *F, double *kernel_h, double *int_w)
+{
+  orc_mesh m;
+  orc_source src;
+  int ierr = orc_mesh_create(sc, &m), k;
+  if (ierr) return ierr;
+  ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll,
+                          orc_src_impl(sc, s), orc_src_h(sc, s), &src);
+  if (!ierr) {
+    if (src.F) {
+      memcpy(F, src.F, sizeof(double) * 2 * (size_t)m.N);
+      if (kernel_h) *kernel_h = src.kernel_h;
+      if (int_w) *int_w = src.int_w;
+    } else {
+      memset(F, 0, sizeof(double) * 2 * (size_t)m.N);
+      for (k = 0; k < m.npe; k++) {
+        long nid = m.element[(size_t)m.npe * src.elem + k];
+        F[2 * nid] += src.vals[2 * k];
+        F[2 * nid + 1] += src.vals[2 * k + 1];
+      }
+      if (kernel_h) *kernel_h = 0.0;
+      if (int_w) *int_w = 0.0;
+    }
+  }
+  free(src.F);
   orc_mesh_free(&m);
   return ierr;
 }
@@ -1117,7 +1267,7 @@ int orc_run_ex(const orc_scenario *sc, int max_steps, const double *u0, const do
   srcs = (orc_source *)calloc(sc->nsrc > 0 ? sc->nsrc : 1, sizeof(orc_source));
   for (s = 0; s < sc->nsrc; s++) {
     ierr = orc_source_setup(&m, &sc->src_xy[2 * s], sc->src_type[s], &sc->src_values[4 * s], sc->use_nearest_internal_gll,
-                            orc_src_impl(sc, s), &srcs[s]);
+                            orc_src_impl(sc, s), orc_src_h(sc, s), &srcs[s]);
     if (ierr) goto done;
   }
   recs = (orc_receiver *)calloc(sc->nrec > 0 ? sc->nrec : 1, sizeof(orc_receiver));
@@ -1142,6 +1292,10 @@ int orc_run_ex(const orc_scenario *sc, int max_steps, const double *u0, const do
     for (s = 0; s < sc->nsrc; s++) {
       double value = orc_stf_eval(sc->stf_type[s], &sc->stf_params[4 * s], time);
       int kk;
+      if (srcs[s].F) { /* an assembled (cubic-spline) source: g += stf(t) F */
+        for (i = 0; i < n2; i++) g[i] += value * srcs[s].F[i];
+        continue;
+      }
       for (kk = 0; kk < m.npe; kk++) {
         long nid = m.element[(size_t)m.npe * srcs[s].elem + kk];
         double b0 = value * srcs[s].vals[2 * kk], b1 = value * srcs[s].vals[2 * kk + 1];
@@ -1175,6 +1329,8 @@ int orc_run_ex(const orc_scenario *sc, int max_steps, const double *u0, const do
   if (a_out) memcpy(a_out, a, sizeof(double) * n2);
   ierr = ORC_OK;
 done:
+  if (srcs)
+    for (s = 0; s < sc->nsrc; s++) free(srcs[s].F);
   free(u); free(v); free(a); free(f); free(g); free(Md); free(febuf); free(row); free(srcs); free(recs); free(uvs);
   orc_mesh_free(&m);
   return ierr;

@@ -994,10 +994,13 @@ int prepare_run(se2w_ctx *c, int32_t nsteps, double dt, const double *psi, hipSt
   HIPCHK(hipSetDevice(c->device));
   if (int ierr = ensure_run_buffers(c, nsteps)) return ierr;
   // Source-time table for this run, through the pinned buffer: the copy is
-  // ordered before the run's kernels on the stream, and the buffer is free
-  // again when the previous run (which synchronised at its end) is done.
+  // ordered before the run's kernels on the stream.  A run waits for its
+  // stream at its end; the wait here covers one that returned early on an
+  // error with the previous copy still queued (on an idle stream it costs a
+  // few microseconds).
   if (c->ns > 0) {
     const size_t n = (size_t)nsteps * c->ns;
+    HIPCHK(hipStreamSynchronize(s));
     if (c->mode == SE2W_FP64) {
       memcpy(c->h_psi, psi, n * sizeof(double));
     } else {
@@ -1587,7 +1590,6 @@ int ens_run(se2w_ensemble *e, int nsteps, const double *dt, const double *const 
   for (size_t i = 0; i < e->members.size(); i++) {
     se2w_ctx *c = e->members[i];
     if (int ierr = prepare_run(c, nsteps, dt[i], psi ? psi[i] : nullptr, e->stream)) return ierr;
-    HIPCHK(hipStreamSynchronize(c->stream));
   }
   if (e->mode == SE2W_FP64) return ens_run_mode<MODE_F64>(e, nsteps, dt);
   if (e->mode == SE2W_FP32) return ens_run_mode<MODE_F32>(e, nsteps, dt);
