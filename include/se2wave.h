@@ -109,7 +109,15 @@ int se2w_stf_create_yoffe(double tau_s, double tau_r, double d_max, SE2WSTF *stf
 #define SE2W_SOURCE_MOMENT 1 /* SOURCE_TYPE_MOMENT */
 #define SE2W_SOURCE_IMPL_POINTWISE 0
 #define SE2W_SOURCE_IMPL_NEAREST_QPOINT 1
+/* SOURCE_IMPL_SPLINE: the cubic-spline smoothed moment source of
+ * AssembleLinearForm_ElastoDynamicsMomentDirac2d_Kernel_CSpline (1925-2057),
+ * spread over every element its kernel (support 2h) reaches. */
+#define SE2W_SOURCE_IMPL_SPLINE 2
 #define SE2W_SOURCE_IMPL_DEFAULT 3
+/* The P0 moment source of AssembleLinearForm_ElastoDynamicsMomentDirac2d_P0
+ * (2059-2198), -source_impl 3 of ElastoDynamicsSetSourceImplementation
+ * (2202-2229); 3 is SOURCE_IMPL_DEFAULT in the SeismicSource numbering. */
+#define SE2W_SOURCE_IMPL_P0 4
 
 typedef struct se2w_source *SE2WSource;
 /* SeismicSourceCreate (2995-3243): locate the element and the nearest
@@ -119,8 +127,27 @@ int se2w_source_create(SE2WSpec spec, int type, int impl, const double coor[2], 
                        int use_nearest_internal_gll, SE2WSource *src);
 int se2w_source_setup(SE2WSource src);    /* SeismicSourceSetup 3245 */
 int se2w_source_destroy(SE2WSource *src); /* SeismicSourceDestroy 3259 */
-/* Element index, closest GLL node and the 2*npe nodal values. */
+/* Element index, closest GLL node and the 2*npe nodal values of a source on
+ * one element.  A cubic-spline source has no single element: elem and
+ * closest_gll are -1 and asking for vals is an error (se2w_source_get_elements
+ * gives its values). */
 int se2w_source_get(SE2WSource src, int32_t *elem, int32_t *closest_gll, double *vals);
+/* Kernel width h of a cubic-spline source: the -sm_h option (1975); h <= 0
+ * keeps the default of 1957-1973 (1.5*(ds/2) for p <= 3, else
+ * 3/(p-2)*(ds/2), ds the larger mesh spacing).  Call before setup. */
+int se2w_source_set_kernel_width(SE2WSource src, double h);
+/* The kernel width in use and the quadrature of W over the mesh (the values
+ * printed at 1971-1978 and 2054) of a set-up cubic-spline source. */
+int se2w_source_get_kernel(SE2WSource src, double *h, double *int_w);
+/* The elements a set-up source injects into: *ne elements elem[*ne] and
+ * their nodal values vals[*ne][2*npe].  A pointwise, nearest-GLL or P0
+ * source has one.  A cubic-spline source assembles its vector over the
+ * elements in ascending order (VecSetValues ADD_VALUES, 2046) and lists each
+ * element that holds a node where that vector is nonzero first in ascending
+ * order: the node's assembled value sits on that element, zero on the
+ * others, so that adding the listed elements' values reproduces the
+ * assembled vector exactly.  With elem = vals = NULL only *ne is set. */
+int se2w_source_get_elements(SE2WSource src, int32_t *ne, int32_t *elem, double *vals);
 /* SeismicSourceEvaluate (3470-3497) on a host vector g[2*nnodes]. */
 int se2w_sources_evaluate(double time, int ns, SE2WSource *src, SE2WSTF *stf, double *g);
 

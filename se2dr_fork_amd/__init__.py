@@ -24,7 +24,7 @@ FP64, FP32, FP32_KAHAN = 0, 1, 2
 KERNEL_LDS, KERNEL_REG, KERNEL_MFMA, KERNEL_LINE, KERNEL_LDS_WIDE, KERNEL_REG64 = 0, 1, 9, 10, 11, 12
 SCHED_FUSED, SCHED_SPLIT = 0, 1
 SOURCE_FORCE, SOURCE_MOMENT = 0, 1
-IMPL_POINTWISE, IMPL_NEAREST_QPOINT, IMPL_DEFAULT = 0, 1, 3
+IMPL_POINTWISE, IMPL_NEAREST_QPOINT, IMPL_SPLINE, IMPL_DEFAULT, IMPL_P0 = 0, 1, 2, 3, 4
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "fp32k": FP32_KAHAN}
 
 
@@ -145,6 +145,9 @@ def _declare(hip, host):
     host.se2w_source_setup.argtypes = [v]
     host.se2w_source_destroy.argtypes = [C.POINTER(v)]
     host.se2w_source_get.argtypes = [v, _ip, _ip, _dp]
+    host.se2w_source_set_kernel_width.argtypes = [v, d]
+    host.se2w_source_get_kernel.argtypes = [v, _dp, _dp]
+    host.se2w_source_get_elements.argtypes = [v, _ip, _ip, _dp]
     host.se2w_sources_evaluate.argtypes = [d, C.c_int, C.POINTER(v), C.POINTER(v), _dp]
     host.se2w_receivers_create.argtypes = [v, C.c_int, _dp, C.POINTER(v)]
     host.se2w_receivers_destroy.argtypes = [C.POINTER(v)]
@@ -333,24 +336,49 @@ class STF:
 
 
 class Source:
-    """SeismicSource: created (located) and set up (nodal values)."""
+    """SeismicSource: created (located) and set up (nodal values).  impl is
+    IMPL_NEAREST_QPOINT, IMPL_POINTWISE, IMPL_SPLINE (the cubic-spline moment
+    source, kernel_h its width as -sm_h sets it, None for the default) or
+    IMPL_P0 (the P0 moment source)."""
 
-    def __init__(self, spec: SpecFE, kind: int, coor, values, impl=IMPL_NEAREST_QPOINT, use_nearest_internal_gll=True):
+    def __init__(self, spec: SpecFE, kind: int, coor, values, impl=IMPL_NEAREST_QPOINT, use_nearest_internal_gll=True,
+                 kernel_h=None):
         _, self._host = libraries()
         self.spec = spec
+        self.impl = impl
         self.h = C.c_void_p()
         c = np.asarray(coor, dtype=np.float64)
         v = np.zeros(4)
         v[: len(values)] = values
         _hchk(self._host.se2w_source_create(spec.h, kind, impl, _ptr(c), _ptr(v), int(bool(use_nearest_internal_gll)),
                                             C.byref(self.h)))
+        if kernel_h is not None:
+            _hchk(self._host.se2w_source_set_kernel_width(self.h, float(kernel_h)))
         _hchk(self._host.se2w_source_setup(self.h))
 
     def get(self):
+        """(element, closest GLL node, 2*npe nodal values) of a one-element source."""
         e, g = C.c_int32(), C.c_int32()
         vals = np.zeros(2 * self.spec.npe)
         _hchk(self._host.se2w_source_get(self.h, C.byref(e), C.byref(g), _ptr(vals)))
         return e.value, g.value, vals
+
+    def elements(self):
+        """(elem[n], vals[n, 2*npe]): the elements the source injects into and
+        their nodal values (one element unless it is a cubic-spline source)."""
+        n = C.c_int32()
+        _hchk(self._host.se2w_source_get_elements(self.h, C.byref(n), None, None))
+        el = np.zeros(max(n.value, 1), dtype=np.int32)
+        vals = np.zeros((max(n.value, 1), 2 * self.spec.npe))
+        _hchk(self._host.se2w_source_get_elements(self.h, C.byref(n), _ptr(el, _ip), _ptr(vals)))
+        return el[: n.value], vals[: n.value]
+
+    def kernel(self):
+        """(h, int W) of a cubic-spline source: its kernel width and the GLL
+        quadrature of the kernel over the mesh."""
+        h, w = C.c_double(), C.c_double()
+        _hchk(self._host.se2w_source_get_kernel(self.h, C.byref(h), C.byref(w)))
+        return h.value, w.value
 
     def close(self):
         if self.h:

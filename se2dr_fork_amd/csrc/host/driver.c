@@ -17,6 +17,11 @@
  *   -src_xy x0,y0,...  -src_type moment|force  -src_values v0..v3 per source
  *   -src_ricker t0,f,amp per source          (replaces the demo sources)
  *   -rec_xy x0,y0,...                          (replaces the demo receivers)
+ *   -source_impl 0|1|2|3   the form of every source, numbered as
+ *                          ElastoDynamicsSetSourceImplementation (2202-2229)
+ *                          numbers them: pointwise, closest internal basis
+ *                          (the demo's), cubic spline, P0; -sm_h h sets the
+ *                          cubic-spline kernel width (1975)
  *   -output_dir DIR
  */
 #include <math.h>
@@ -36,6 +41,26 @@
       return ierr_;                                                               \
     }                                                                             \
   } while (0)
+
+/* -source_impl (ElastoDynamicsSourceSetup, 2238-2242): the reference's
+ * numbering 0..3 to the SeismicSource implementation, with the lines
+ * ElastoDynamicsSetSourceImplementation prints (2211-2222). */
+static int source_impl_option(SE2WOptions opts, int *impl)
+{
+  int v = 1, found = 0;
+  *impl = SE2W_SOURCE_IMPL_NEAREST_QPOINT;
+  CHK(se2w_options_get_int(opts, "-source_impl", &v, &found));
+  if (!found) return 0;
+  printf("  [source implementation]: processed from command line args\n");
+  switch (v) {
+    case 0: printf("  [source implementation]: pointwise evaluation\n"); *impl = SE2W_SOURCE_IMPL_POINTWISE; break;
+    case 1: printf("  [source implementation]: closest internal basis\n"); *impl = SE2W_SOURCE_IMPL_NEAREST_QPOINT; break;
+    case 2: printf("  [source implementation]: cspline kernel\n"); *impl = SE2W_SOURCE_IMPL_SPLINE; break;
+    case 3: printf("  [source implementation]: P0\n"); *impl = SE2W_SOURCE_IMPL_P0; break;
+    default: fprintf(stderr, "[se2wave] -source_impl %d: valid choices are 0,1,2,3\n", v); return SE2W_HOST_ERR_SUP;
+  }
+  return 0;
+}
 
 /* CHK for the step loop: a snapshot still being written is waited for
    before the error returns (its thread reads the GPU context). */
@@ -203,7 +228,8 @@ static int run_demo(SE2WOptions opts, int mx, int my)
   double scale[2] = {2.0e3, 2.0e3}, shift[2] = {-1.0e3, -1.0e3}, arr[4 * MAX_SRC];
   double dt, time_max = 0.4, cfl_factor = 0.2, *xr, *times = NULL, *trace = NULL;
   int p = 2, nsources, nrecv, nt, of = 5000, found, n, s, k, prec, sched, use_internal = 1;
-  int dump_ic_src = 0, ignore_receiver_output = 0, nsteps_total;
+  int dump_ic_src = 0, ignore_receiver_output = 0, nsteps_total, impl, found_h = 0;
+  double sm_h = 0.0;
   char outdir[512] = ".";
   SE2WVtkJob vtk_job;
 
@@ -214,6 +240,8 @@ static int run_demo(SE2WOptions opts, int mx, int my)
   CHK(se2w_options_get_bool(opts, "-use_nearest_internal_gll", &use_internal, NULL));
   CHK(se2w_options_get_string(opts, "-output_dir", outdir, sizeof(outdir), NULL));
   CHK(parse_precision(opts, &prec, &sched));
+  CHK(source_impl_option(opts, &impl));
+  CHK(se2w_options_get_real(opts, "-sm_h", &sm_h, &found_h));
   CHK(se2w_options_get_int(opts, "-bdegree", &p, NULL));
   CHK(se2w_spec_create(mx, my, p, &spec));
   n = 2;
@@ -239,9 +267,9 @@ static int run_demo(SE2WOptions opts, int mx, int my)
     const double c1[2] = {400.1, 400.1}, m1[4] = {0.0, -1000.0, 1000.0, 0.0};
     nsources = 2;
     printf("Moment: [ %+1.2e , %+1.2e ; %+1.2e , %+1.2e ]\n", m0[0], m0[1], m0[2], m0[3]);
-    CHK(se2w_source_create(spec, SE2W_SOURCE_MOMENT, SE2W_SOURCE_IMPL_NEAREST_QPOINT, c0, m0, use_internal, &src[0]));
+    CHK(se2w_source_create(spec, SE2W_SOURCE_MOMENT, impl, c0, m0, use_internal, &src[0]));
     printf("Moment: [ %+1.2e , %+1.2e ; %+1.2e , %+1.2e ]\n", m1[0], m1[1], m1[2], m1[3]);
-    CHK(se2w_source_create(spec, SE2W_SOURCE_MOMENT, SE2W_SOURCE_IMPL_NEAREST_QPOINT, c1, m1, use_internal, &src[1]));
+    CHK(se2w_source_create(spec, SE2W_SOURCE_MOMENT, impl, c1, m1, use_internal, &src[1]));
     CHK(se2w_stf_create_ricker(0.15, 12.0, 1.0, &stf[0]));
     CHK(se2w_stf_create_ricker(0.25, 4.0, 1.0, &stf[1]));
   } else {
@@ -259,15 +287,26 @@ static int run_demo(SE2WOptions opts, int mx, int my)
       return SE2W_HOST_ERR_ARG;
     }
     for (s = 0; s < nsources; s++) {
-      CHK(se2w_source_create(spec, type, SE2W_SOURCE_IMPL_NEAREST_QPOINT, &arr[2 * s], &vals[4 * s], use_internal, &src[s]));
+      CHK(se2w_source_create(spec, type, impl, &arr[2 * s], &vals[4 * s], use_internal, &src[s]));
       CHK(se2w_stf_create_ricker(ricker[3 * s], ricker[3 * s + 1], ricker[3 * s + 2], &stf[s]));
     }
   }
   for (s = 0; s < nsources; s++) {
     int32_t e, g;
+    if (impl == SE2W_SOURCE_IMPL_SPLINE && found_h) CHK(se2w_source_set_kernel_width(src[s], sm_h));
     CHK(se2w_source_setup(src[s]));
     CHK(se2w_source_get(src[s], &e, &g, NULL));
     printf("[SeismicSource] source --> element %d closest_gll %d\n", (int)e, (int)g);
+    if (impl == SE2W_SOURCE_IMPL_SPLINE) { /* the lines of 1971, 1977 and 2054 */
+      double h, int_w;
+      int32_t ne;
+      CHK(se2w_source_get_kernel(src[s], &h, &int_w));
+      CHK(se2w_source_get_elements(src[s], &ne, NULL, NULL));
+      if (found_h) printf("kernel_h = %1.4e <from options>\n", h);
+      else if (p > 3) printf("kernel_h = %1.4e <with pfac>\n", h);
+      printf("\\int W = %1.14e \n", int_w);
+      printf("[SeismicSource] cubic-spline source over %d elements\n", (int)ne);
+    }
   }
   if (dump_ic_src) { /* 4663-4669: the sources with STF = 1 */
     const int64_t nn = (int64_t)(p * mx + 1) * (p * my + 1);

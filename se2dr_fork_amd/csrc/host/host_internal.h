@@ -49,6 +49,12 @@ struct se2w_source {
   double coor[2], xi[2], values[4];
   int elem, closest_gll, issetup;
   double vals[2 * HOST_MAX_NB * HOST_MAX_NB];
+  /* cubic-spline source: kernel width (requested, in use), \int W, and the
+     elements it injects into with their values [nsub][2*npe] */
+  double h_opt, kernel_h, int_w;
+  int nsub;
+  int32_t *sub_elem;
+  double *sub_vals;
 };
 
 struct se2w_receivers {

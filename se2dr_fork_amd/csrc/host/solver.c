@@ -17,6 +17,10 @@ struct se2w_solver {
   int ns;            /* sources handed to the GPU (those with a source and an STF) */
   SE2WSource *src;   /* [ns] */
   SE2WSTF *stf;      /* [ns] */
+  int ngpu;          /* element sources on the GPU: one per source, one per element of a cubic-spline source */
+  int *parent;       /* [ngpu] the source each belongs to (its column of the source-time table) */
+  int one_each;      /* every source on exactly one element: parent[j] = j */
+  double *stfv;      /* [ns] the STF values of one step */
   SE2WReceivers recv;
   double time;
   int steps;
@@ -73,12 +77,28 @@ int se2w_solver_create(SE2WSpec spec, int precision, int schedule, int ns, SE2WS
   if ((ierr = se2w_spec_mass(spec, mass))) goto bad;
   if ((ierr = se2w_set_mass(sv->ctx, mass))) { ierr = gpu_fail(ierr); goto bad; }
   if (sv->ns > 0) {
-    elem = (int32_t *)malloc(sizeof(int32_t) * sv->ns);
-    vals = (double *)malloc(sizeof(double) * 2 * spec->npe * sv->ns);
-    if (!elem || !vals) { ierr = host_fail(SE2W_HOST_ERR_MEM, "out of memory"); goto bad; }
-    for (k = 0; k < sv->ns; k++)
-      if ((ierr = se2w_source_get(sv->src[k], &elem[k], NULL, &vals[(size_t)2 * spec->npe * k]))) goto bad;
-    if ((ierr = se2w_set_sources(sv->ctx, sv->ns, elem, vals))) { ierr = gpu_fail(ierr); goto bad; }
+    /* every source as element sources in source order; the GPU adds them in
+       that order, each with its source's STF value */
+    int j = 0;
+    for (k = 0; k < sv->ns; k++) {
+      int32_t ne;
+      if ((ierr = se2w_source_get_elements(sv->src[k], &ne, NULL, NULL))) goto bad;
+      sv->ngpu += ne;
+    }
+    elem = (int32_t *)malloc(sizeof(int32_t) * (sv->ngpu > 0 ? sv->ngpu : 1));
+    vals = (double *)malloc(sizeof(double) * 2 * spec->npe * (sv->ngpu > 0 ? sv->ngpu : 1));
+    sv->parent = (int *)malloc(sizeof(int) * (sv->ngpu > 0 ? sv->ngpu : 1));
+    sv->stfv = (double *)malloc(sizeof(double) * sv->ns);
+    if (!elem || !vals || !sv->parent || !sv->stfv) { ierr = host_fail(SE2W_HOST_ERR_MEM, "out of memory"); goto bad; }
+    for (k = 0; k < sv->ns; k++) {
+      int32_t ne, q;
+      if ((ierr = se2w_source_get_elements(sv->src[k], &ne, &elem[j], &vals[(size_t)2 * spec->npe * j]))) goto bad;
+      for (q = 0; q < ne; q++) sv->parent[j + q] = k;
+      j += ne;
+    }
+    sv->one_each = sv->ngpu == sv->ns;
+    for (k = 0; k < sv->ngpu && sv->one_each; k++) sv->one_each = sv->parent[k] == k;
+    if (sv->ngpu > 0 && (ierr = se2w_set_sources(sv->ctx, sv->ngpu, elem, vals))) { ierr = gpu_fail(ierr); goto bad; }
   }
   if (recv && recv->nr > 0) {
     if ((ierr = se2w_set_receivers(sv->ctx, recv->nr, recv->elem, recv->gll))) { ierr = gpu_fail(ierr); goto bad; }
@@ -107,7 +127,7 @@ int se2w_solver_destroy(SE2WSolver *sp)
 {
   if (!sp || !*sp) return 0;
   if ((*sp)->ctx) se2w_destroy((*sp)->ctx);
-  free((*sp)->src); free((*sp)->stf);
+  free((*sp)->src); free((*sp)->stf); free((*sp)->parent); free((*sp)->stfv);
   free((*sp)->times); free((*sp)->trace); free((*sp)->psi);
   free((*sp)->uv_weights); free((*sp)->uv_trace);
   free(*sp);
@@ -173,10 +193,26 @@ int se2w_solver_reserve(SE2WSolver sv, int nsteps)
   int ierr;
   if (!sv || nsteps < 1) return host_fail(SE2W_HOST_ERR_ARG, "bad reserve arguments");
   if ((ierr = grow(&sv->times, &sv->times_cap, (size_t)nsteps))) return ierr;
-  if (sv->ns > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ns))) return ierr;
+  if (sv->ngpu > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ngpu))) return ierr;
   if (nr > 0 && (ierr = grow(&sv->trace, &sv->trace_cap, (size_t)nsteps * nr * 7))) return ierr;
   if (sv->nuv > 0 && (ierr = grow(&sv->uv_trace, &sv->uv_cap, (size_t)nsteps * sv->nuv * 4))) return ierr;
   if ((ierr = se2w_reserve(sv->ctx, nsteps))) return gpu_fail(ierr);
+  return 0;
+}
+
+/* One row of the GPU's source-time table: each source's STF at time t
+ * (SeismicSourceEvaluate, 3470-3497), repeated for each of its elements. */
+static int stf_row(struct se2w_solver *sv, double t, double *row)
+{
+  int s, j, ierr;
+  if (sv->one_each) {
+    for (s = 0; s < sv->ns; s++)
+      if ((ierr = se2w_stf_evaluate(sv->stf[s], t, &row[s]))) return ierr;
+    return 0;
+  }
+  for (s = 0; s < sv->ns; s++)
+    if ((ierr = se2w_stf_evaluate(sv->stf[s], t, &sv->stfv[s]))) return ierr;
+  for (j = 0; j < sv->ngpu; j++) row[j] = sv->stfv[sv->parent[j]];
   return 0;
 }
 
@@ -195,15 +231,14 @@ int se2w_solver_advance(SE2WSolver sv, int nsteps, double dt)
   int k, s, ierr;
   if (!sv || nsteps < 1) return host_fail(SE2W_HOST_ERR_ARG, "bad advance arguments");
   if ((ierr = grow(&sv->times, &sv->times_cap, (size_t)nsteps))) return ierr;
-  if (sv->ns > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ns))) return ierr;
+  if (sv->ngpu > 0 && (ierr = grow(&sv->psi, &sv->psi_cap, (size_t)nsteps * sv->ngpu))) return ierr;
   if (nr > 0 && (ierr = grow(&sv->trace, &sv->trace_cap, (size_t)nsteps * nr * 7))) return ierr;
   t = sv->time;
   tp[0] = now_ms();
   for (k = 0; k < nsteps; k++) {
     t = t + dt;
     sv->times[k] = t;
-    for (s = 0; s < sv->ns; s++)
-      if ((ierr = se2w_stf_evaluate(sv->stf[s], t, &sv->psi[(size_t)k * sv->ns + s]))) return ierr;
+    if ((ierr = stf_row(sv, t, &sv->psi[(size_t)k * sv->ngpu]))) return ierr;
   }
   tp[1] = now_ms();
   if ((ierr = se2w_run(sv->ctx, nsteps, dt, sv->psi))) return gpu_fail(ierr);
@@ -251,15 +286,14 @@ int se2w_solvers_advance(int n, SE2WSolver *sv, se2w_ensemble *ens, int nsteps, 
     const int nr = v->recv ? v->recv->nr : 0;
     double t = v->time;
     if ((ierr = grow(&v->times, &v->times_cap, (size_t)nsteps))) goto done;
-    if (v->ns > 0 && (ierr = grow(&v->psi, &v->psi_cap, (size_t)nsteps * v->ns))) goto done;
+    if (v->ngpu > 0 && (ierr = grow(&v->psi, &v->psi_cap, (size_t)nsteps * v->ngpu))) goto done;
     if (nr > 0 && (ierr = grow(&v->trace, &v->trace_cap, (size_t)nsteps * nr * 7))) goto done;
     for (k = 0; k < nsteps; k++) {
       t = t + dt[i];
       v->times[k] = t;
-      for (s = 0; s < v->ns; s++)
-        if ((ierr = se2w_stf_evaluate(v->stf[s], t, &v->psi[(size_t)k * v->ns + s]))) goto done;
+      if ((ierr = stf_row(v, t, &v->psi[(size_t)k * v->ngpu]))) goto done;
     }
-    psi[i] = v->ns > 0 ? v->psi : NULL;
+    psi[i] = v->ngpu > 0 ? v->psi : NULL;
   }
   if ((ierr = se2w_ensemble_run(ens, nsteps, dt, psi))) { ierr = gpu_fail(ierr); goto done; }
   for (i = 0; i < n; i++) {

@@ -15,7 +15,7 @@ from typing import List, Optional, Tuple
 
 import numpy as np
 
-from . import (FP64, IMPL_NEAREST_QPOINT, SCHED_FUSED, SOURCE_FORCE, SOURCE_MOMENT, STF, Receivers, Solver, Source,
+from . import (FP64, IMPL_NEAREST_QPOINT, IMPL_SPLINE, SCHED_FUSED, SOURCE_FORCE, SOURCE_MOMENT, STF, Receivers, Solver, Source,
                SpecFE, _ptr, libraries)
 
 
@@ -40,7 +40,8 @@ class Scenario:
     tmax: float = 1.0e30
     use_nearest_internal_gll: bool = True
     dt: Optional[float] = None              # explicit dt overrides cfl_factor * CFL
-    src_impl: Optional[List[int]] = None    # per source IMPL_NEAREST_QPOINT (default) or IMPL_POINTWISE
+    src_impl: Optional[List[int]] = None    # per source IMPL_NEAREST_QPOINT (default), IMPL_POINTWISE, IMPL_SPLINE, IMPL_P0
+    src_h: Optional[List[float]] = None     # per source cubic-spline kernel width (-sm_h); <= 0 or None: the default
 
     @property
     def nnodes(self) -> int:
@@ -197,8 +198,9 @@ def build(sc: Scenario, precision=FP64, schedule=SCHED_FUSED, with_solver=True) 
     sources, stfs = [], []
     for s in range(len(sc.src_type)):
         impl = IMPL_NEAREST_QPOINT if sc.src_impl is None else sc.src_impl[s]
+        kh = None if (sc.src_h is None or impl != IMPL_SPLINE) else sc.src_h[s]
         sources.append(Source(spec, sc.src_type[s], sc.src_xy[s], sc.src_values[s], impl=impl,
-                              use_nearest_internal_gll=sc.use_nearest_internal_gll))
+                              use_nearest_internal_gll=sc.use_nearest_internal_gll, kernel_h=kh))
         kind, a, b, c = sc.stf[s]
         stfs.append(STF(kind, a, b, c))
     rec = Receivers(spec, sc.rec_xy)

@@ -71,6 +71,29 @@ static int host_layer(const char *dir)
   EXPECT(se2w_source_create(spec, SE2W_SOURCE_MOMENT, SE2W_SOURCE_IMPL_POINTWISE, &xr[0], mom, 0, &src[2]) == 0);
   for (k = 0; k < 3; k++) EXPECT(se2w_source_setup(src[k]) == 0 && se2w_source_get(src[k], &e, &cg, vals) == 0);
   EXPECT(se2w_sources_evaluate(0.025, 3, src, stf, g) == 0);
+  /* the legacy cubic-spline (default and set width) and P0 forms */
+  {
+    SE2WSource leg[3] = {NULL, NULL, NULL};
+    int32_t ne = 0, *lel;
+    double h, int_w, *lv;
+    EXPECT(se2w_source_create(spec, SE2W_SOURCE_MOMENT, SE2W_SOURCE_IMPL_SPLINE, &xr[0], mom, 1, &leg[0]) == 0);
+    EXPECT(se2w_source_create(spec, SE2W_SOURCE_MOMENT, SE2W_SOURCE_IMPL_SPLINE, &xr[2], mom, 1, &leg[1]) == 0);
+    EXPECT(se2w_source_set_kernel_width(leg[1], 900.0) == 0);
+    EXPECT(se2w_source_create(spec, SE2W_SOURCE_MOMENT, SE2W_SOURCE_IMPL_P0, &xr[2], mom, 1, &leg[2]) == 0);
+    for (k = 0; k < 3; k++) {
+      EXPECT(se2w_source_setup(leg[k]) == 0 && se2w_source_get_elements(leg[k], &ne, NULL, NULL) == 0 && ne >= 1);
+      lel = (int32_t *)malloc(sizeof(int32_t) * ne);
+      lv = (double *)malloc(sizeof(double) * ne * 2 * npe);
+      EXPECT(lel && lv && se2w_source_get_elements(leg[k], &ne, lel, lv) == 0);
+      free(lel);
+      free(lv);
+    }
+    EXPECT(se2w_source_get_kernel(leg[1], &h, &int_w) == 0 && h == 900.0);
+    EXPECT(se2w_source_get(leg[0], &e, &cg, vals) != 0 && se2w_source_get(leg[2], &e, &cg, vals) == 0);
+    EXPECT(se2w_sources_evaluate(0.025, 3, leg, stf, g) == 0);
+    EXPECT(se2w_source_set_kernel_width(leg[1], 100.0) != 0); /* after setup */
+    for (k = 0; k < 3; k++) se2w_source_destroy(&leg[k]);
+  }
   EXPECT(se2w_receivers_create(spec, 2, xr, &rec) == 0 && se2w_receivers_get(rec, nid, el, gll) == 0);
   EXPECT(se2w_receivers_write_header(rec, dir) == 0);
   for (k = 0; k < 3 * 2 * 7; k++) rows[k] = 1.0e-3 * k;
@@ -133,10 +156,12 @@ static int oracle_run(const char *dir)
    * per-step file output, RecordUV and RecordUV_interp stations */
   const int mx = 5, my = 4, ne = 20;
   double lam[20], mu[20], rho[20], src_xy[4] = {1230.0, 2111.0, 3001.0, 901.0};
-  int src_type[2] = {1, 0}, stf_type[2] = {0, 1}, src_impl[2] = {0, 1}, uv_interp[3] = {0, 1, 1};
-  double src_values[8] = {1e9, 2e8, -3e8, 7e8, 2e9, 1e10, 0.0, 0.0};
-  double stf_params[8] = {0.02, 25.0, 1.0, 0.0, 0.03, 40.0, 2.0, 0.0};
+  int src_type[4] = {1, 0, 1, 1}, stf_type[4] = {0, 1, 0, 1}, src_impl[4] = {0, 1, 2, 4}, uv_interp[3] = {0, 1, 1};
+  double src_values[16] = {1e9, 2e8, -3e8, 7e8, 2e9, 1e10, 0.0, 0.0, 1e9, 0.0, 0.0, 1e9, 0.0, 5e8, 5e8, 0.0};
+  double stf_params[16] = {0.02, 25.0, 1.0, 0.0, 0.03, 40.0, 2.0, 0.0, 0.02, 25.0, 1.0, 0.0, 0.03, 40.0, 2.0, 0.0};
+  double src_h[4] = {0.0, 0.0, 700.0, 0.0}, Fa[2 * 21 * 17], kh, iw;
   double rec_xy[4] = {1500.3, 2200.7, 3999.0, 3999.0}, uv_xy[6] = {1500.3, 2200.7, 1500.3, 2200.7, 10.0, 3990.0};
+  double src_xy4[8] = {1230.0, 2111.0, 3001.0, 901.0, 2000.0, 2000.0, 3700.0, 3600.0};
   double trace[30 * 2 * 7], times[30], uvt[30 * 3 * 4], *u, *F;
   orc_scenario sc;
   int k, done = 0;
@@ -150,13 +175,15 @@ static int oracle_run(const char *dir)
   sc.mx = mx; sc.my = my; sc.p = 4;
   sc.scale[0] = sc.scale[1] = 4000.0;
   sc.lambda = lam; sc.mu = mu; sc.rho = rho;
-  sc.nsrc = 2; sc.src_xy = src_xy; sc.src_type = src_type; sc.src_values = src_values;
+  sc.nsrc = 4; sc.src_xy = src_xy4; sc.src_type = src_type; sc.src_values = src_values; sc.src_h = src_h;
   sc.stf_type = stf_type; sc.stf_params = stf_params; sc.use_nearest_internal_gll = 1;
   sc.nrec = 2; sc.rec_xy = rec_xy;
   sc.dt = 1.0e-3; sc.nt = 30; sc.tmax = 1.0e30; sc.nthreads = 2;
   sc.src_impl = src_impl; sc.nuv = 3; sc.uv_xy = uv_xy; sc.uv_interp = uv_interp; sc.rec_dir = dir;
   EXPECT(orc_run_ex(&sc, 0, NULL, NULL, NULL, trace, times, &done, NULL, NULL, NULL, uvt) == 0 && done == 30);
   EXPECT(orc_write_receiver_files(&sc, dir, done, times, trace) == 0);
+  for (k = 0; k < 4; k++) EXPECT(orc_source_assembled(&sc, k, Fa, &kh, &iw) == 0);
+  (void)src_xy;
   u = (double *)calloc(n2, sizeof(double));
   F = (double *)calloc(n2, sizeof(double));
   EXPECT(u && F);

@@ -53,3 +53,14 @@ def test_verify_stf_needs_a_choice(tmp_path):
 def test_bad_options_are_rejected(tmp_path):
     r = run(["-verify_stf", "-stf_ricker", "-tmax", "abc"], tmp_path)
     assert r.returncode != 0 and "not a real number" in r.stderr
+
+
+def test_source_impl_option_errors(tmp_path):
+    """-source_impl takes the reference's numbering 0..3 (se2wave.c:2202-2229);
+    the cubic-spline and P0 forms assemble moment sources only."""
+    r = run(["-mx", "4", "-source_impl", "7"], tmp_path)
+    assert r.returncode != 0 and "valid choices are 0,1,2,3" in r.stderr
+    r = run(["-mx", "4", "-source_impl", "3", "-src_xy", "10,10", "-src_type", "force", "-src_values", "1,0,0,0",
+             "-src_ricker", "0.1,10,1"], tmp_path)
+    assert r.returncode != 0 and "assemble moment sources" in r.stderr
+    assert "[source implementation]: P0" in r.stdout

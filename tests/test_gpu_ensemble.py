@@ -77,6 +77,28 @@ def test_fp64_ensemble_bit_identical_to_oracle(oracle, p):
     ens.close()
 
 
+def test_fp64_ensemble_with_cubic_spline_and_p0_sources(oracle):
+    """Members whose sources span several elements (the cubic-spline form)
+    or sit on one cell (P0): each member's table of source-time values is
+    widened to its element sources, bit-identical to the oracle."""
+    scs = uneven(4, n=3, seed=500)
+    impls = ([S.IMPL_SPLINE], [S.IMPL_P0, S.IMPL_SPLINE], [S.IMPL_NEAREST_QPOINT, S.IMPL_SPLINE, S.IMPL_P0])
+    for sc, im in zip(scs, impls):
+        sc.src_type = [S.SOURCE_MOMENT] * len(sc.src_type)
+        sc.src_values = np.tile([1.3, -0.4, 0.7, 2.1], (len(sc.src_type), 1))
+        sc.src_impl = im
+    builts, ens = ensemble(scs, S.FP64)
+    got = ens_runs(builts, ens, 24, chunks=(11, 13))
+    for sc, b, (tr, u, v, a) in zip(scs, builts, got):
+        pr = oracle.Problem(sc, dt=b.dt, nt=24)
+        _, tr0, u0, v0, a0 = pr.run(max_steps=24)
+        np.testing.assert_array_equal(tr, tr0)
+        np.testing.assert_array_equal(u, u0)
+        np.testing.assert_array_equal(v, v0)
+        np.testing.assert_array_equal(a, a0)
+    ens.close()
+
+
 @pytest.mark.parametrize("prec", [S.FP32, S.FP32_KAHAN])
 @pytest.mark.parametrize("p", [4, 6, 8])
 def test_fp32_ensemble_equals_single_runs_for_every_variant(prec, p):

@@ -52,6 +52,22 @@ def small_cases():
         sc.src_impl = [S.IMPL_POINTWISE] * 3
         sc.name = "pointwise"
         yield sc
+    yield from legacy_cases()
+
+
+def legacy_cases():
+    """Cubic-spline (se2wave.c:1925-2057) and P0 (2059-2198) moment sources
+    beside a nearest-node one; the wide kernel (-sm_h) spans tile borders."""
+    I = S
+    for p, mx, my, impls, h in ((3, 13, 11, (I.IMPL_SPLINE, I.IMPL_P0, I.IMPL_NEAREST_QPOINT), None),
+                                (6, 9, 8, (I.IMPL_SPLINE, I.IMPL_SPLINE, I.IMPL_P0), None),
+                                (4, 37, 19, (I.IMPL_P0, I.IMPL_SPLINE, I.IMPL_SPLINE), [0.0, 0.0, 390.0]),
+                                (2, 8, 8, (I.IMPL_SPLINE, I.IMPL_SPLINE), [700.0, 0.0])):
+        sc = SC.small(mx=mx, my=my, p=p, nt=30, nsrc=len(impls), nrec=6, seed=80 + p)
+        sc.src_impl = list(impls)
+        sc.src_h = h
+        sc.name = "legacy"
+        yield sc
 
 
 @pytest.mark.parametrize("sc", list(small_cases()), ids=lambda s: f"{s.name}-{s.mx}x{s.my}-p{s.p}")
@@ -108,6 +124,17 @@ def test_fp64_golden_fixtures(name):
     np.testing.assert_array_equal(u, z["u"])
     np.testing.assert_array_equal(v, z["v"])
     np.testing.assert_array_equal(a, z["a"])
+
+
+def test_legacy_sources_fp32_error():
+    """FP32 modes with cubic-spline and P0 sources against the FP64 run."""
+    sc = list(legacy_cases())[2]
+    sc.nt = 200
+    _, _, rows, _, _, _ = gpu_run(sc)
+    for prec, tol_uv in [(S.FP32_KAHAN, 1e-5), (S.FP32, 3e-5)]:
+        _, _, r32, _, _, _ = gpu_run(sc, precision=prec)
+        assert rel_l2(r32[..., 0:2], rows[..., 0:2]) < tol_uv  # displacement
+        assert rel_l2(r32[..., 2:4], rows[..., 2:4]) < tol_uv  # velocity
 
 
 def test_config1_fp64_matches_golden_and_fp32_error():
@@ -380,6 +407,40 @@ def test_program_demo_run_matches_oracle(oracle, tmp_path):
         for f, ref in (("disp", uo), ("velo", vo)):
             np.testing.assert_array_equal(a[f + "_x"], ref[0::2])
             np.testing.assert_array_equal(a[f + "_y"], ref[1::2])
+
+
+@pytest.mark.parametrize("source_impl, sm_h", [(2, None), (2, 90.0), (3, None)])
+def test_program_legacy_source_forms_match_oracle(oracle, tmp_path, source_impl, sm_h):
+    """-source_impl 2 (cubic spline, -sm_h its width) and 3 (P0), numbered as
+    ElastoDynamicsSetSourceImplementation numbers them (se2wave.c:2202-2229),
+    applied to the demo's sources: the receiver file equals the oracle's and
+    the printed kernel lines carry the oracle's width and \\int W."""
+    exe = os.path.join(ROOT, "se2dr_fork_amd", "bin", "se2wave")
+    args = ["-mx", "12", "-bdegree", "4", "-tmax", "0.03", "-cfl_factor", "0.5", "-of", "100000",
+            "-source_impl", str(source_impl), "-output_dir", str(tmp_path)]
+    if sm_h is not None:
+        args += ["-sm_h", str(sm_h)]
+    r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stderr
+    sc = SC.demo(mx=12, p=4, nt=100000, tmax=0.03, cfl_factor=0.5)
+    impl = S.IMPL_SPLINE if source_impl == 2 else S.IMPL_P0
+    sc.src_impl = [impl] * len(sc.src_type)
+    sc.src_h = None if sm_h is None else [sm_h] * len(sc.src_type)
+    b = SC.build(sc, with_solver=False)
+    pr = oracle.Problem(sc, dt=b.dt, nt=b.nsteps)
+    if impl == S.IMPL_SPLINE:
+        for s in range(len(sc.src_type)):
+            _, h, int_w = pr.assembled_source(s)
+            assert ("\\int W = %1.14e \n" % int_w) in r.stdout
+            if sm_h is not None:
+                assert h == sm_h and ("kernel_h = %1.4e <from options>" % h) in r.stdout
+            else:
+                assert ("kernel_h = %1.4e <with pfac>" % h) in r.stdout
+    to, ro, _, _, _ = pr.run()
+    (tmp_path / "orc").mkdir()
+    pr.write_receiver_files(str(tmp_path / "orc"), to, ro)
+    name = "closestqpsource-receiverCP-uva-12x12-p4-rank0.dat"
+    assert (tmp_path / name).read_bytes() == (tmp_path / "orc" / name).read_bytes()
 
 
 @pytest.mark.parametrize("prec", [S.FP64, S.FP32_KAHAN])

@@ -143,6 +143,99 @@ def test_host_sources_evaluate_matches_gpu_convention():
     np.testing.assert_array_equal(g, ref)
 
 
+def legacy_scenario(p, impls, h=None, seed=40, mx=9, my=8):
+    sc = SC.small(mx=mx, my=my, p=p, seed=seed + p, nsrc=len(impls), nrec=3)
+    sc.src_type = [S.SOURCE_MOMENT] * len(impls)
+    sc.src_values = np.array([[1.3, -0.4, 0.7, 2.1], [-0.2, 0.9, 1.1, 0.3], [0.5, 0.5, -1.5, 0.8]])[: len(impls)]
+    sc.src_impl = list(impls)
+    sc.src_h = h
+    return sc
+
+
+def element_sum(sc, el, vals):
+    """Adds element values into a nodal vector in list order."""
+    F = np.zeros(2 * sc.nnodes)
+    nb, nx = sc.p + 1, sc.p * sc.mx + 1
+    for e, v in zip(el, vals):
+        ei, ej = e % sc.mx, e // sc.mx
+        for k in range(nb * nb):
+            n = (ei * sc.p + k % nb) + (ej * sc.p + k // nb) * nx
+            F[2 * n] += v[2 * k]
+            F[2 * n + 1] += v[2 * k + 1]
+    return F
+
+
+@pytest.mark.parametrize("p", [1, 2, 3, 4, 6, 8])
+def test_legacy_sources_bit_identical_to_oracle(oracle, p):
+    """The cubic-spline source's per-element values (se2wave.c:1925-2057),
+    added in list order, are the oracle's assembled vector bit for bit, with
+    each nonzero node value on exactly one element; the P0 source
+    (2059-2198) is one element's values; the kernel width and \\int W agree."""
+    impls = [S.IMPL_SPLINE, S.IMPL_P0, S.IMPL_SPLINE]
+    for h in (None, [0.0, 0.0, 517.0]):
+        sc = legacy_scenario(p, impls, h=h)
+        if p == 1:
+            sc.use_nearest_internal_gll = False
+        b = SC.build(sc, with_solver=False)
+        pr = oracle.Problem(sc, dt=b.dt)
+        for s, src in enumerate(b.sources):
+            F0, h0, w0 = pr.assembled_source(s)
+            el, vals = src.elements()
+            assert list(el) == sorted(el)
+            np.testing.assert_array_equal(element_sum(sc, el, vals), F0)
+            if impls[s] == S.IMPL_SPLINE:
+                assert src.kernel() == (h0, w0)
+                assert len(el) > 1 and np.count_nonzero(vals) == np.count_nonzero(F0)
+                with pytest.raises(S.Se2wError, match="spans"):
+                    src.get()
+            else:
+                eo, go, vo = pr.source(s)
+                ep, gp, vp = src.get()
+                assert (eo, go) == (ep, gp) and list(el) == [ep]
+                np.testing.assert_array_equal(vo, vp)
+                with pytest.raises(S.Se2wError, match="kernel width"):
+                    src.kernel()
+
+
+def test_host_sources_evaluate_adds_the_assembled_vectors(oracle):
+    """SeismicSourceEvaluate with cubic-spline, P0 and nearest-node sources:
+    g = sum_s stf_s(t) F_s in source order, bit-identical to the oracle's
+    assembled vectors scaled the same way."""
+    sc = legacy_scenario(4, [S.IMPL_SPLINE, S.IMPL_NEAREST_QPOINT, S.IMPL_P0])
+    b = SC.build(sc, with_solver=False)
+    pr = oracle.Problem(sc, dt=b.dt)
+    _, host = S.libraries()
+    import ctypes as C
+
+    g = np.zeros(2 * sc.nnodes)
+    ns = len(b.sources)
+    srcs = (C.c_void_p * ns)(*[s.h for s in b.sources])
+    stfs = (C.c_void_p * ns)(*[s.h for s in b.stfs])
+    assert host.se2w_sources_evaluate(0.047, ns, srcs, stfs, S._ptr(g)) == 0
+    ref = np.zeros_like(g)
+    for s in range(ns):
+        F = pr.assembled_source(s)[0]
+        ref = ref + b.stfs[s](0.047) * F
+    np.testing.assert_array_equal(g, ref)
+
+
+def test_legacy_source_error_paths():
+    spec = S.SpecFE(6, 6, 3)
+    spec.scale_coords((600.0, 600.0), (0.0, 0.0))
+    spec.set_constant_material_velocity(5000.0, 3000.0, 2500.0)
+    for impl in (S.IMPL_SPLINE, S.IMPL_P0):
+        with pytest.raises(S.Se2wError, match="moment sources"):
+            S.Source(spec, S.SOURCE_FORCE, (300.0, 300.0), (1, 0), impl=impl)
+        with pytest.raises(S.Se2wError, match="Source x-coordinate"):
+            S.Source(spec, S.SOURCE_MOMENT, (-1.0, 300.0), (1, 0, 0, 1), impl=impl)
+    with pytest.raises(S.Se2wError, match="only a cubic-spline source"):
+        S.Source(spec, S.SOURCE_MOMENT, (300.0, 300.0), (1, 0, 0, 1), impl=S.IMPL_P0, kernel_h=50.0)
+    with pytest.raises(S.Se2wError, match="not supported"):
+        S.Source(spec, S.SOURCE_MOMENT, (300.0, 300.0), (1, 0, 0, 1), impl=5)
+    src = S.Source(spec, S.SOURCE_MOMENT, (300.0, 300.0), (1, 0, 0, 1), impl=S.IMPL_SPLINE, kernel_h=50.0)
+    assert src.kernel()[0] == 50.0
+
+
 def test_rand48_matches_the_c_library():
     """The perturbed model's generator is POSIX drand48 (PETSc's PETSCRAND48
     type calls srand48/drand48): the restatement must give libc's bits."""

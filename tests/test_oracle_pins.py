@@ -389,6 +389,134 @@ def test_pointwise_source_on_a_node_equals_the_nearest_node_source(oracle):
 
 
 # --------------------------------------------------------------------------
+# Legacy assembled moment sources (se2wave.c:1925-2198)
+# --------------------------------------------------------------------------
+def _moment_scenario(p, impl, mx=9, my=8, xy=((1900.0, 2100.0), (2650.0, 1375.0)), h=None, seed=21):
+    import se2dr_fork_amd as S
+
+    sc = _scenario(mx=mx, my=my, p=p, seed=seed, nsrc=len(xy))
+    sc.src_xy = np.array(xy, dtype=float)
+    sc.src_type = [S.SOURCE_MOMENT] * len(xy)
+    sc.src_values = np.array([[1.3, -0.4, 0.7, 2.1], [-0.2, 0.9, 1.1, 0.3]])[: len(xy)]
+    sc.src_impl = [impl] * len(xy)
+    sc.src_h = None if h is None else [h] * len(xy)
+    return sc
+
+
+def _linear_field(c, L=4000.0):
+    """u = (a + b x + c y, d + e x + f y) on the nodes and its gradient."""
+    x, y = c[0::2] / L, c[1::2] / L
+    B = np.array([[0.7, -1.9], [2.3, 0.4]]) / L  # du_i/dx_j
+    return 0.3 + B[0, 0] * x * L + B[0, 1] * y * L, -1.1 + B[1, 0] * x * L + B[1, 1] * y * L, B
+
+
+@pytest.mark.parametrize("p", [2, 3, 4, 6, 8])
+def test_cspline_source_kernel_width_and_mass(oracle, p):
+    """Kernel_CSpline (se2wave.c:1925-2057): h = 1.5 (ds/2) for p <= 3 and
+    3/(p-2) (ds/2) above, ds the larger spacing (1939-1973), -sm_h replaces
+    it (1975); W integrates to 1 over the plane, so the printed \\int W of a
+    source well inside the mesh is 1 to quadrature accuracy."""
+    import se2dr_fork_amd as S
+
+    sc = _moment_scenario(p, S.IMPL_SPLINE, mx=9, my=8)
+    ds = max(4000.0 / 9, 4000.0 / 8)
+    want = 1.5 * 0.5 * ds if p <= 3 else 3.0 / (p - 2.0) * 0.5 * ds
+    pr = oracle.Problem(sc, dt=1e-5)
+    for s in range(2):
+        F, h, int_w = pr.assembled_source(s)
+        assert h == pytest.approx(want, rel=1e-15)
+        assert abs(int_w - 1.0) < 1.5e-2
+    pr = oracle.Problem(_moment_scenario(p, S.IMPL_SPLINE, mx=9, my=8, h=333.0), dt=1e-5)
+    assert pr.assembled_source(0)[1] == 333.0
+    # a kernel narrower than the node spacing around a point between nodes
+    # reaches no GLL point: the source vanishes
+    pr = oracle.Problem(_moment_scenario(p, S.IMPL_SPLINE, h=1.0, xy=((1901.3, 2101.7),)), dt=1e-5)
+    F, _, int_w = pr.assembled_source(0)
+    assert int_w == 0.0 and not F.any()
+
+
+@pytest.mark.parametrize("p", [2, 3, 5, 8])
+def test_cspline_source_closed_forms(oracle, p):
+    """For a linear displacement field u the assembled vector gives
+    sum_n F_n . u_n = (M : grad u) \\int W exactly (sum_i grad N_i(x_q) u_i =
+    grad u for a linear u at every GLL point), which a transposed moment
+    tensor, swapped derivatives or a missing Jacobian fails; F vanishes on
+    every node of an element lying wholly beyond the support 2h, and the
+    kernel's value at its centre is 10 / (7 pi h^2)."""
+    import se2dr_fork_amd as S
+
+    sc = _moment_scenario(p, S.IMPL_SPLINE)
+    pr = oracle.Problem(sc, dt=1e-5)
+    c, _, _ = pr.coords()
+    ux, uy, B = _linear_field(c)
+    for s in range(2):
+        F, h, int_w = pr.assembled_source(s)
+        Mxx, Mxy, Myx, Myy = sc.src_values[s]
+        lhs = np.dot(F[0::2], ux) + np.dot(F[1::2], uy)
+        rhs = (Mxx * B[0, 0] + Mxy * B[0, 1] + Myx * B[1, 0] + Myy * B[1, 1]) * int_w
+        assert lhs == pytest.approx(rhs, rel=1e-11)
+        # support: a node all of whose elements lie wholly beyond 2h carries
+        # nothing, and the elements reaching inside 2h carry the whole vector
+        xs, ys = sc.src_xy[s]
+        hx, hy = 4000.0 / sc.mx, 4000.0 / sc.my
+        reached = np.zeros(len(ux), dtype=bool)
+        for e in range(sc.mx * sc.my):
+            ex, ey = e % sc.mx, e // sc.mx
+            dx = max(ex * hx - xs, 0.0, xs - (ex + 1) * hx)
+            dy = max(ey * hy - ys, 0.0, ys - (ey + 1) * hy)
+            if math.hypot(dx, dy) < 2.0 * h:
+                reached[_element_nodes(sc, e)] = True
+        assert not F[0::2][~reached].any() and not F[1::2][~reached].any()
+        assert np.count_nonzero(F[0::2][reached]) > 0.5 * reached.sum()
+    # W(0) through a one-point kernel: h just above the distance to nodes other
+    # than the source node makes every other GLL point fall outside 2h
+    node = _element_nodes(sc, 40)[p + 1 + 1]
+    xs, ys = c[2 * node], c[2 * node + 1]
+    others = np.hypot(c[0::2] - xs, c[1::2] - ys)
+    h = 0.49 * np.min(others[others > 0])
+    pr = oracle.Problem(_moment_scenario(p, S.IMPL_SPLINE, xy=((xs, ys),), h=h), dt=1e-5)
+    _, _, int_w = pr.assembled_source(0)
+    xi, w = oracle.gll(p)
+    wq = w[1] * w[1] * (0.5 * 4000.0 / sc.mx) * (0.5 * 4000.0 / sc.my)  # the node's w_q detJ
+    assert int_w == pytest.approx(wq * 10.0 / (7.0 * math.pi * h * h), rel=1e-14)
+
+
+@pytest.mark.parametrize("p", [2, 4, 7])
+def test_p0_source_closed_forms(oracle, p):
+    """P0 (se2wave.c:2059-2198): the basis gradients at the centre of the cell
+    holding the source, weighted by sum_q w_q detJ / (dx dy) = 1 on a uniform
+    mesh, so sum_k vals_k . u_k = M : grad u for linear u and the values equal
+    those of a pointwise source placed at the cell centre."""
+    import se2dr_fork_amd as S
+
+    sc = _moment_scenario(p, S.IMPL_P0)
+    pr = oracle.Problem(sc, dt=1e-5)
+    c, _, _ = pr.coords()
+    ux, uy, B = _linear_field(c)
+    hx, hy = 4000.0 / sc.mx, 4000.0 / sc.my
+    for s in range(2):
+        e, g, vals = pr.source(s)
+        xs, ys = sc.src_xy[s]
+        assert e == int(xs / hx) + int(ys / hy) * sc.mx and g == -1
+        nodes = _element_nodes(sc, e)
+        Mxx, Mxy, Myx, Myy = sc.src_values[s]
+        lhs = np.dot(vals[0::2], ux[nodes]) + np.dot(vals[1::2], uy[nodes])
+        assert lhs == pytest.approx(Mxx * B[0, 0] + Mxy * B[0, 1] + Myx * B[1, 0] + Myy * B[1, 1], rel=1e-11)
+        # equals the pointwise source at the cell centre
+        ex, ey = e % sc.mx, e // sc.mx
+        centre = ((ex + 0.5) * hx, (ey + 0.5) * hy)
+        pc = _moment_scenario(p, S.IMPL_POINTWISE, xy=(centre,))
+        pc.src_values = sc.src_values[s : s + 1]
+        e2, _, v2 = oracle.Problem(pc, dt=1e-5).source(0)
+        assert e2 == e
+        np.testing.assert_allclose(vals, v2, rtol=1e-12, atol=1e-13 * np.abs(v2).max())
+        F, h, int_w = pr.assembled_source(s)
+        assert h == 0.0 and int_w == 0.0
+        np.testing.assert_array_equal(F[2 * nodes], vals[0::2])
+        np.testing.assert_array_equal(F[2 * nodes + 1], vals[1::2])
+
+
+# --------------------------------------------------------------------------
 # Whole time loop
 # --------------------------------------------------------------------------
 def test_mirror_symmetry_of_explosive_source(oracle):
