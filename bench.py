@@ -495,9 +495,9 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
                 b.solver.ctx.set_kernel(v)
                 _, tr = b.solver.advance(full_steps, b.dt)
                 row["receiver_rel_l2_vs_fp64"] = field_errors(tr, ref)
-                u, v, _ = b.solver.ctx.get_state()
-                row["state_rel_l2_vs_fp64"] = {"u": rel_l2(u, ref_state[0]), "v": rel_l2(v, ref_state[1])}
-                del u, v
+                su, sv, _ = b.solver.ctx.get_state()
+                row["state_rel_l2_vs_fp64"] = {"u": rel_l2(su, ref_state[0]), "v": rel_l2(sv, ref_state[1])}
+                del su, sv
             out["variants"].append(row)
             log(f"variants: {name} variant {v} ({row['kernel']}) {row['kernel_ms']:.4f} ms")
             b.solver.close()
