@@ -218,7 +218,9 @@ int se2w_material_layered(int mx, int my, int nlayers, uint64_t seed, double vp_
 /* ---------------------------------------------------------------------- */
 typedef struct se2w_solver *SE2WSolver;
 /* Builds the GPU context from the spec, sources (with their STFs) and
- * receivers; the objects must outlive the solver. */
+ * receivers; the objects must outlive the solver.  Each source goes to the
+ * GPU as its elements (se2w_source_get_elements), in source order, and each
+ * step's table holds the source's STF value once per element. */
 int se2w_solver_create(SE2WSpec spec, int precision, int schedule, int ns, SE2WSource *src, SE2WSTF *stf,
                        SE2WReceivers recv, SE2WSolver *solver);
 int se2w_solver_destroy(SE2WSolver *solver);

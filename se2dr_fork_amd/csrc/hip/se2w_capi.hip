@@ -1288,7 +1288,9 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
       variant != SE2W_KERNEL_LINE && variant != SE2W_KERNEL_LDS_WIDE && !is_reg64_variant(variant))
     return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
   if (is_reg64_variant(variant) && !reg64_variant_built(c->p, c->mode, variant, c->schedule))
-    return fail(SE2W_ERR_UNSUPPORTED, "the FP64 register-blocked kernel needs FP64, 2 <= p <= %d and the fused schedule",
+    return fail(SE2W_ERR_UNSUPPORTED,
+                "the FP64 register-blocked kernel needs FP64, 2 <= p <= %d (a lane pair per element above p = 4) "
+                "and the fused schedule",
                 kReg64MaxP);
   if (variant == SE2W_KERNEL_LDS_WIDE && c->schedule != SE2W_SCHED_FUSED)
     return fail(SE2W_ERR_UNSUPPORTED, "the 512-thread tile kernel is built for the fused schedule");

@@ -42,10 +42,12 @@ constexpr Reg64Schedule kReg64Schedules[] = {{32, 4, 3, 0, 2}, {32, 4, 0, 0, 2},
                                              {32, 4, 2, 0, 1}, {64, 2, 2, 0, 1}, {32, 4, 2, 4, 1}};
 constexpr int kNumReg64Schedules = sizeof(kReg64Schedules) / sizeof(kReg64Schedules[0]);
 // Degrees with a register-resident element (two (p+1)^2 arrays of doubles
-// per lane): 2 <= p <= 4; fused schedule only.
-constexpr int kReg64MaxP = 4;
+// per lane): 2 <= p <= 6, fused schedule only; above p = 4 only the lane
+// pairs (one lane per element would need over 300 VGPRs at p = 5).
+constexpr int kReg64MaxP = 6;
 constexpr bool reg64_schedule_built(int P, int S, bool fused) {
-  return fused && P >= 2 && P <= kReg64MaxP && S >= 0 && S < kNumReg64Schedules;
+  return fused && P >= 2 && P <= kReg64MaxP && S >= 0 && S < kNumReg64Schedules &&
+         (P <= 4 || kReg64Schedules[S].lpe == 2);
 }
 
 // The value that lane 2k (x) or lane 2k+1 (y) of a pair holds, in both

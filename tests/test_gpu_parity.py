@@ -85,12 +85,13 @@ FP64_VARIANTS = [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in
 
 
 @pytest.mark.parametrize("p, mx, my", [(2, 9, 7), (3, 33, 5), (4, 41, 9), (4, 3, 2), (4, 70, 13), (4, 40, 1),
-                                        (4, 1, 17)])
+                                        (4, 1, 17), (5, 23, 11), (6, 33, 9), (6, 2, 17)])
 def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
-    """Every FP64 step kernel (LDS tile kernels, register kernel schedules)
-    runs the reference's operation order: receiver rows and the whole state
-    bit-identical to the oracle, on meshes of one tile, several tiles with
-    partial edge tiles, one element row and one element column."""
+    """Every FP64 step kernel (LDS tile kernels, register kernel schedules;
+    above p = 4 the lane-pair schedules only) runs the reference's operation
+    order: receiver rows and the whole state bit-identical to the oracle, on
+    meshes of one tile, several tiles with partial edge tiles, one element
+    row and one element column."""
     sc = SC.small(mx=mx, my=my, p=p, nt=30, seed=100 + p + mx, nsrc=2, nrec=5)
     b0 = SC.build(sc)
     pr = oracle.Problem(sc, dt=b0.dt, nt=b0.nsteps)
@@ -101,7 +102,7 @@ def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
         try:
             b, times, rows, u, v, a = gpu_run(sc, kernel=k, chunks=2)
         except S.Se2wError:
-            assert k >= S.KERNEL_REG64 and p > 4
+            assert k >= S.KERNEL_REG64 + 4 and p > 4  # one lane per element: p <= 4
             continue
         ran += 1
         np.testing.assert_array_equal(rows, ro, err_msg=f"variant {k}")
@@ -109,7 +110,7 @@ def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
         np.testing.assert_array_equal(v, vo, err_msg=f"variant {k}")
         np.testing.assert_array_equal(a, ao, err_msg=f"variant {k}")
         b.solver.close()
-    assert ran == len(FP64_VARIANTS)
+    assert ran == len(FP64_VARIANTS) - (3 if p > 4 else 0)
 
 
 @pytest.mark.parametrize("name", ["small_p3", "small_p4_force", "small_p8"])
