@@ -147,6 +147,9 @@ def ens_kernel_signature(variant, p, mode):
         return f"ens_line_kernel<{p}, {mode}>"
     if variant == 11:
         return f"ens_tile_kernel<{p}, {mode}, 512>"
+    if variant >= KERNEL_REG64:
+        ex, ey, w, pf, lpe = REG64_SCHEDULES[variant - KERNEL_REG64]
+        return f"ens_reg64_kernel<{p}, {ex}, {ey}, {w}, {pf}, {lpe}>"
     ex, ey, w, pf, lpe = REG_SCHEDULES[variant - 1]
     return f"ens_reg_kernel<{p}, {mode}, {ex}, {ey}, {w}, {pf}, {lpe}>"
 
@@ -302,11 +305,14 @@ def cpu_baseline(sc, dt, seconds_target=8.0):
             "cpu_model": model, "nproc": nproc, "usable_cpus": usable}
 
 
-def ensemble_bench(S, SC, n=256, nt=1500, n_fp64=16):
+def ensemble_bench(S, SC, n=256, nt=1500):
     """The uneven ensemble workload: n instances of mx, my ~ U{64..192}, p = 6,
     FP32, per-cell random materials, one Ricker source, 8 receivers, nt steps
     each; all advanced together by one batched launch per step (HIP graph),
-    against the same n instances solved one after another."""
+    against the same n instances solved one after another.  Beside it, the
+    plain FP64 method on the same instances, batched the same way (each
+    member bit-identical to its own FP64 solve): its throughput, and the
+    receiver error of every FP32 member against it."""
     scs = [SC.config3_instance(i, nt=nt) for i in range(n)]
     nodes = sum(sc.nnodes for sc in scs)
     builts = [SC.build(sc, precision=S.FP32) for sc in scs]
@@ -334,14 +340,23 @@ def ensemble_bench(S, SC, n=256, nt=1500, n_fp64=16):
         t_seq += time.perf_counter() - t0
         errs.append(rel_l2(tr[:, :, :4], res[i][1][:, :, :4]))
         b.solver.close()
-    # receiver error of the batched FP32 members against FP64 solves of the
-    # same instances (a sample of the ensemble)
-    log(f"ensemble: FP64 solves of {min(n_fp64, n)} members")
-    e64 = {"u": [], "v": []}
-    for i in range(min(n_fp64, n)):
-        b = SC.build(scs[i], precision=S.FP64)
-        _, tr = b.solver.advance(nt, b.dt)
+    # the plain FP64 method on the same instances, batched
+    log(f"ensemble: {n} members in FP64, batched run")
+    builts = [SC.build(sc, precision=S.FP64) for sc in scs]
+    ens = S.Ensemble([b.solver for b in builts])
+    best64, _ = ens.autotune(steps=5, reps=2)
+    for b in builts:
+        b.solver.reserve(nt)
+    t0 = time.perf_counter()
+    res64 = ens.advance(nt, dts)
+    t_ens64 = time.perf_counter() - t0
+    _, step_ms64 = ens.timing()
+    ens.close()
+    for b in builts:
         b.solver.close()
+    e64 = {"u": [], "v": []}
+    for i in range(n):
+        tr = res64[i][1]
         e64["u"].append(rel_l2(res[i][1][:, :, 0:2], tr[:, :, 0:2]))
         e64["v"].append(rel_l2(res[i][1][:, :, 2:4], tr[:, :, 2:4]))
     return {"workload": f"{n} instances, mx, my ~ U{{64..192}}, p=6, FP32, per-cell random material, "
@@ -353,6 +368,8 @@ def ensemble_bench(S, SC, n=256, nt=1500, n_fp64=16):
             "kernel": ens_kernel_signature(best, 6, 1),
             "tiles": tiles, "grid": grid,
             "max_rel_l2_u_v_vs_sequential": max(errs),
+            "fp64": {"solves_per_s": n / t_ens64, "ensemble_s": t_ens64, "batched_step_ms": step_ms64,
+                     "node_updates_per_s": nodes * nt / t_ens64, "kernel": ens_kernel_signature(best64, 6, 0)},
             "receiver_rel_l2_vs_fp64": {"members": len(e64["u"]),
                                         "u_max": max(e64["u"]), "u_median": float(np.median(e64["u"])),
                                         "v_max": max(e64["v"]), "v_median": float(np.median(e64["v"]))}}

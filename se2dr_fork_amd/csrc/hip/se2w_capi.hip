@@ -1681,7 +1681,7 @@ int se2w_ensemble_autotune(se2w_ensemble *e, int32_t steps, int32_t reps, int32_
   if (!e || steps < 1 || reps < 1) return fail(SE2W_ERR_ARG, "bad autotune arguments");
   HIPCHK(hipSetDevice(e->device));
   std::vector<int> cands;
-  for (int v = 0; v <= SE2W_KERNEL_LDS_WIDE; v++)
+  for (int v = 0; v < SE2W_KERNEL_REG64 + kNumReg64Schedules; v++)
     if (ens_variant_built(e, v)) cands.push_back(v);
   const size_t n = e->members.size();
   std::vector<Snapshot> sn(n);

@@ -6,6 +6,7 @@
 
 #include "se2w.h"
 #include "se2w_kernels_reg.hpp"
+#include "se2w_kernels_reg64.hpp"
 #include "se2w_kernels_mfma.hpp"
 
 #ifndef SE2W_MODE_TU
@@ -60,6 +61,15 @@ void ens_reg_kernel(EnsArgs<float> a) {
   });
 }
 
+template <int P, int EX, int EY, int WAVES, int PF, int LPE>
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+void ens_reg64_kernel(EnsArgs<double> a) {
+  ens_tile(a, [](auto &prm, int tile, int step, int last) {
+    if constexpr (LPE == 2) reg64_body<P, EX, EY, PF>(prm, tile, step, last);
+    else reg64s_body<P, EX, EY, PF>(prm, tile, step, last);
+  });
+}
+
 template <int P, int MODE>
 __global__ __launch_bounds__(kThreads) void ens_mfma_kernel(EnsArgs<float> a) {
   ens_tile(a, [](auto &prm, int tile, int step, int last) { mfma_body<P, MODE, false>(prm, tile, step, last); });
@@ -96,6 +106,30 @@ int reg_dispatch(int sched, const void *args, int grid, hipStream_t s, int *bpc,
   return -1;
 }
 
+template <int P, int S>
+int reg64_dispatch(int sched, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
+  if constexpr (S < kNumReg64Schedules) {
+    if constexpr (reg64_schedule_built(P, S, true)) {
+      if (sched == S) {
+        constexpr Reg64Schedule r = kReg64Schedules[S];
+        auto k = ens_reg64_kernel<P, r.ex, r.ey, r.waves, r.pf, r.lpe>;
+        const int nt = r.ex * r.ey * r.lpe;
+        if (!args) {
+          int n = 0;
+          if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k, nt, 0) != hipSuccess) return -1;
+          *bpc = n;
+          *thr = nt;
+          return 0;
+        }
+        hipLaunchKernelGGL(k, dim3(grid), dim3(nt), 0, s, *(const EnsArgs<double> *)args);
+        return 0;
+      }
+    }
+    return reg64_dispatch<P, S + 1>(sched, args, grid, s, bpc, thr);
+  }
+  return -1;
+}
+
 template <int P, int MODE>
 int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc, int *thr) {
   using R = typename RealOf<MODE>::T;
@@ -128,6 +162,8 @@ int dispatch_pm(int variant, const void *args, int grid, hipStream_t s, int *bpc
   }
   if constexpr (MODE != MODE_F64 && P >= 2)
     return reg_dispatch<P, MODE, 0>(variant - SE2W_KERNEL_REG, args, grid, s, bpc, thr);
+  if constexpr (MODE == MODE_F64 && P >= 2)
+    return reg64_dispatch<P, 0>(variant - SE2W_KERNEL_REG64, args, grid, s, bpc, thr);
   return -1;
 }
 

@@ -35,6 +35,8 @@ def test_kernel_signatures():
     ex, ey, w, pf, lpe = bench.REG_SCHEDULES[6]
     assert bench.kernel_signature(7, 4, 2) == f"reg_kernel<4, 2, true, {ex}, {ey}, {w}, {pf}, {lpe}>"
     assert bench.ens_kernel_signature(4, 6, 1).startswith("ens_reg_kernel<6, 1, ")
+    ex, ey, w, pf, lpe = bench.REG64_SCHEDULES[1]
+    assert bench.ens_kernel_signature(bench.KERNEL_REG64 + 1, 6, 0) == f"ens_reg64_kernel<6, {ex}, {ey}, {w}, {pf}, {lpe}>"
 
 
 def test_committed_result_line():

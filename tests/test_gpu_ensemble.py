@@ -77,6 +77,32 @@ def test_fp64_ensemble_bit_identical_to_oracle(oracle, p):
     ens.close()
 
 
+@pytest.mark.parametrize("p", [4, 6])
+def test_fp64_ensemble_every_variant_bit_identical_to_oracle(oracle, p):
+    """Every FP64 batched kernel (the LDS tile kernels and the register
+    kernel schedules built for p) advances each member exactly as the
+    oracle does."""
+    scs = uneven(p, n=3, seed=600 + p)
+    want = []
+    for sc in scs:
+        b = SC.build(sc, precision=S.FP64, with_solver=False)
+        _, tr0, u0, v0, a0 = oracle.Problem(sc, dt=b.dt, nt=20).run(max_steps=20)
+        want.append((tr0, u0, v0, a0))
+    tried = 0
+    for var in [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in range(7)]:
+        builts, ens = ensemble(scs, S.FP64)
+        try:
+            ens.set_kernel(var)
+        except S.Se2wError:
+            ens.close()
+            assert var >= S.KERNEL_REG64 + 4 and p > 4  # one lane per element: p <= 4
+            continue
+        tried += 1
+        assert_same(ens_runs(builts, ens, 20, chunks=(7, 13)), want)
+        ens.close()
+    assert tried == (9 if p <= 4 else 6)
+
+
 def test_fp64_ensemble_with_cubic_spline_and_p0_sources(oracle):
     """Members whose sources span several elements (the cubic-spline form)
     or sit on one cell (P0): each member's table of source-time values is

@@ -254,6 +254,49 @@ def test_state_roundtrip_and_restart(oracle):
     np.testing.assert_array_equal(b.solver.ctx.get_state()[0], uo)
 
 
+@pytest.mark.parametrize("prec", [S.FP64, S.FP32_KAHAN])
+def test_edge_cases_against_the_oracle(oracle, prec):
+    """Free vibration with no sources and no receivers (a nonzero initial
+    state, run in single steps and then a chunk); one element with sources
+    on the domain's corner and left edge and receivers on two corners
+    (element location clamped at the upper bound, 3035-3048, 2736-2760);
+    one element row and one element column.  FP64 bit-identical to the
+    oracle, FP32+Kahan within 1e-5."""
+    def check(got, want, tol):
+        for g, w in zip(got, want):
+            if prec == S.FP64:
+                np.testing.assert_array_equal(g, w)
+            elif w.size:
+                assert rel_l2(g, w) < tol
+
+    sc = SC.small(mx=5, my=3, p=3, nsrc=0, nrec=0, nt=20)
+    sc.src_xy, sc.src_values, sc.rec_xy = np.zeros((0, 2)), np.zeros((0, 4)), np.zeros((0, 2))
+    rng = np.random.default_rng(11)
+    u0, v0 = rng.standard_normal(2 * sc.nnodes) * 1e-3, rng.standard_normal(2 * sc.nnodes) * 1e-2
+    a0 = np.zeros_like(u0)
+    b = SC.build(sc, precision=prec)
+    b.solver.ctx.set_state(u0, v0, a0)
+    rows = [b.solver.advance(1, b.dt)[1] for _ in range(5)] + [b.solver.advance(15, b.dt)[1]]
+    assert all(r.size == 0 for r in rows)
+    _, _, uo, vo, ao = oracle.Problem(sc, dt=b.dt, nt=20).run(u0=u0, v0=v0, a0=a0)
+    check(b.solver.ctx.get_state(), (uo, vo, ao), 1e-5)
+    b.solver.close()
+
+    corner = SC.small(mx=1, my=1, p=4, nsrc=2, nrec=2, nt=40)
+    corner.src_xy = np.array([[4000.0, 4000.0], [0.0, 2000.0]])
+    corner.rec_xy = np.array([[0.0, 0.0], [4000.0, 4000.0]])
+    row = SC.small(mx=19, my=1, p=2, nsrc=1, nrec=3, nt=40, seed=12)
+    col = SC.small(mx=1, my=23, p=5, nsrc=1, nrec=3, nt=40, seed=13)
+    for sc in (corner, row, col):
+        b, _, rows, u, v, a = gpu_run(sc, precision=prec, chunks=3)
+        _, ro, uo, vo, ao = oracle.Problem(sc, dt=b.dt, nt=b.nsteps).run()
+        check((u, v), (uo, vo), 1e-5)
+        if prec == S.FP64:
+            np.testing.assert_array_equal(rows, ro)
+            np.testing.assert_array_equal(a, ao)
+        b.solver.close()
+
+
 def test_assemble_forces(oracle):
     for sc in [SC.small(mx=9, my=7, p=4), SC.small(mx=5, my=6, p=7, seed=9)]:
         u = np.random.default_rng(sc.p).standard_normal(2 * sc.nnodes)
