@@ -479,6 +479,12 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
     (HIP events), node-updates/s, and with full_steps > 0 the receiver error
     of each variant after full_steps steps against the FP64 run."""
     sc = SC.config2(nt=max(steps, full_steps, 1), mx=mx, p=p) if p != 8 or mx != 2048 else SC.config5(nt=max(steps, full_steps, 1))
+    near = 0
+    if full_steps > 0 and p == 8 and mx == 2048:
+        # config 5's receiver line sees no wave in its 500 steps: add one
+        # receiver beside each source (sources peaking at t = 0) for the check
+        sc = SC.with_near_source_receivers(sc)
+        near = len(sc.src_xy)
     hip, _ = S.libraries()
     out = {"mx": mx, "p": p, "nodes": sc.nnodes, "steps": steps, "variants": []}
     ref = ref_state = None
@@ -511,7 +517,9 @@ def kernel_variant_bench(S, SC, mx, p, steps, precisions=("fp32",), full_steps=0
                 b = SC.build(sc, precision=prec)
                 b.solver.ctx.set_kernel(v)
                 _, tr = b.solver.advance(full_steps, b.dt)
-                row["receiver_rel_l2_vs_fp64"] = field_errors(tr, ref)
+                row["receiver_rel_l2_vs_fp64"] = field_errors(tr[:, : tr.shape[1] - near], ref[:, : ref.shape[1] - near])
+                if near:
+                    row["near_source_receiver_rel_l2_vs_fp64"] = field_errors(tr[:, -near:], ref[:, -near:])
                 su, sv, _ = b.solver.ctx.get_state()
                 row["state_rel_l2_vs_fp64"] = {"u": rel_l2(su, ref_state[0]), "v": rel_l2(sv, ref_state[1])}
                 del su, sv

@@ -166,6 +166,19 @@ def config5(nt=500, mx=2048, p=8, seed=5) -> Scenario:
                     rho=rho, src_xy=xy, src_type=ty, src_values=vals, stf=stf, rec_xy=rec, cfl_factor=0.2, nt=nt)
 
 
+def with_near_source_receivers(sc: Scenario, offsets=((0.7, 0.0), (0.0, 1.9), (-2.6, 1.1), (2.4, -2.5))) -> Scenario:
+    """The scenario with its Ricker sources peaking at t = 0 and one extra
+    receiver a few metres from each source (appended after its own
+    receivers), so that a short run carries signal to those receivers (config
+    5's 500 steps simulate 0.7 ms, in which no wave reaches its receiver
+    line)."""
+    ns = len(sc.src_xy)
+    sc.stf = [("ricker", 0.0, f, a) for (_, _, f, a) in sc.stf]
+    off = np.array([offsets[s % len(offsets)] for s in range(ns)])
+    sc.rec_xy = np.concatenate([np.asarray(sc.rec_xy).reshape(-1, 2), sc.src_xy + off])
+    return sc
+
+
 def small(mx=6, my=5, p=3, seed=11, nt=40, kind=SOURCE_MOMENT, nsrc=2, nrec=4) -> Scenario:
     """A small heterogeneous case for parity tests (uneven mx, my)."""
     vp, vs, rho = random_cells(mx, my, seed, 4000.0, 6000.0, 2500.0, 3000.0)

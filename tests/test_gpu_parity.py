@@ -533,12 +533,11 @@ def test_config2_full_size(oracle):
 
 def _config5_with_signal():
     """The 2048x2048, p=8 layered workload (268M nodes) with its four Ricker
-    sources peaking at t = 0 and one receiver 0.7-3.5 m from each source, so
-    that 500 steps (0.7 ms simulated) carry signal to every receiver."""
-    sc = SC.config5()
-    sc.stf = [("ricker", 0.0, f, a) for (_, _, f, a) in sc.stf]
-    off = np.array([[0.7, 0.0], [0.0, 1.9], [-2.6, 1.1], [2.4, -2.5]])
-    sc.rec_xy = sc.src_xy + off
+    sources peaking at t = 0 and only the receivers 0.7-3.5 m from each
+    source, so that 500 steps (0.7 ms simulated) carry signal to every
+    receiver."""
+    sc = SC.with_near_source_receivers(SC.config5())
+    sc.rec_xy = sc.rec_xy[-len(sc.src_xy):]
     return sc
 
 
