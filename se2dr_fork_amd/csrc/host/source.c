@@ -201,6 +201,10 @@ static int setup_spline(struct se2w_source *src)
   if (src->h_opt > 0.0) h = src->h_opt;
   src->kernel_h = h;
   src->int_w = 0.0;
+  free(src->sub_elem); /* a setup that failed part-way may have left them */
+  free(src->sub_vals);
+  src->sub_elem = NULL;
+  src->sub_vals = NULL;
   band(s->xcol, s->mx, p, xs[0], 2.02 * h, &i0, &i1);
   band(s->yrow, s->my, p, xs[1], 2.02 * h, &j0, &j1);
   src->nsub = 0;
