@@ -225,7 +225,8 @@ def split_schedule_bench(S, SC, sc, precs, steps=100, warmup=10):
         variant = sv.ctx.kernel()
         sv.close()
         row = {"steps": steps, "ms_per_step": run_ms / steps,
-               "element_kernel": kernel_signature(variant, sc.p, prec).replace(", true", ", false"),
+               "element_kernel": kernel_signature(variant, sc.p, prec).replace(", true", ", false")
+                                 .replace("reg64_kernel<", "reg64_element_kernel<"),
                "update_kernel": f"update_kernel<{sc.p}, {prec}>",
                "element_kernel_ms": float(kern[0]), "update_kernel_ms": float(kern[1]),
                "element_bytes_per_node": SPLIT_ELEMENT_BYTES[prec], "update_bytes_per_node": SPLIT_UPDATE_BYTES[prec]}

@@ -1122,7 +1122,8 @@ namespace {
 
 // The element kernel that f = K(u) runs on: the context's variant when that
 // variant has an element-only form (the LDS tile kernel, or the register
-// kernel's first schedule), else the LDS tile kernel.  The fused-only
+// kernel's first schedule), in FP64 the FP64 register kernel's element-only
+// form where it is built, else the LDS tile kernel.  The fused-only
 // variants (the other register schedules, the 512-thread tile, matrix-core
 // and line kernels) tile the mesh differently from the element-only kernel
 // they would fall back to, so the call switches the tiling with the kernel.
@@ -1130,6 +1131,7 @@ int element_only_variant(const se2w_ctx *c) {
   if (is_reg_variant(c->variant) && reg_available(c->p, c->mode) &&
       reg_variant_built(c->p, c->variant, SE2W_SCHED_SPLIT))
     return c->variant;
+  if (reg64_variant_built(c->p, c->mode, SE2W_KERNEL_REG64, SE2W_SCHED_SPLIT)) return SE2W_KERNEL_REG64;
   return SE2W_KERNEL_LDS;
 }
 

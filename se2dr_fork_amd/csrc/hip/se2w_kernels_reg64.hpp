@@ -42,12 +42,14 @@ constexpr Reg64Schedule kReg64Schedules[] = {{32, 4, 3, 0, 2}, {32, 4, 0, 0, 2},
                                              {32, 4, 2, 0, 1}, {64, 2, 2, 0, 1}, {32, 4, 2, 4, 1}};
 constexpr int kNumReg64Schedules = sizeof(kReg64Schedules) / sizeof(kReg64Schedules[0]);
 // Degrees with a register-resident element (two (p+1)^2 arrays of doubles
-// per lane): 2 <= p <= 6, fused schedule only; above p = 4 only the lane
-// pairs (one lane per element would need over 300 VGPRs at p = 5).
+// per lane): 2 <= p <= 6; above p = 4 only the lane pairs (one lane per
+// element would need over 300 VGPRs at p = 5).  Every schedule for the fused
+// step, the first one also as the element kernel of the split schedule
+// (f = K(u) stored, no update).
 constexpr int kReg64MaxP = 6;
 constexpr bool reg64_schedule_built(int P, int S, bool fused) {
-  return fused && P >= 2 && P <= kReg64MaxP && S >= 0 && S < kNumReg64Schedules &&
-         (P <= 4 || kReg64Schedules[S].lpe == 2);
+  return P >= 2 && P <= kReg64MaxP && S >= 0 && S < kNumReg64Schedules && (P <= 4 || kReg64Schedules[S].lpe == 2) &&
+         (fused || S == 0);
 }
 
 // The value that lane 2k (x) or lane 2k+1 (y) of a pair holds, in both
@@ -67,7 +69,7 @@ __device__ __forceinline__ double &comp64(double2 &v, int c) { return reinterpre
 // One workgroup of EX x EY x 2 threads processes one tile of (EX-1) x (EY-1)
 // elements (all threads call it with the same tile).  A workgroup that calls
 // it for several tiles in a row puts a barrier between the calls.
-template <int P, int EX, int EY, int PF, typename PRM>
+template <int P, int EX, int EY, int PF, bool FUSED = true, typename PRM>
 __device__ __forceinline__ void reg64_body(PRM &prm, int tile, int step, int last) {
 #pragma clang fp contract(off)
   constexpr int NB = P + 1;
@@ -252,7 +254,7 @@ __device__ __forceinline__ void reg64_body(PRM &prm, int tile, int step, int las
   __syncthreads();
   const TileLists tl = tile_lists(prm, tile, step, last);
   constexpr int NSL = RPW * CPL, CHUNK = 8;
-  if constexpr (PF == 0) {
+  if constexpr (FUSED && PF == 0) {
 #pragma unroll
     for (int r = 0; r < RPW; r++)
 #pragma unroll
@@ -291,11 +293,14 @@ __device__ __forceinline__ void reg64_body(PRM &prm, int tile, int step, int las
       const int gx = ex0 * P + lxn, gy = ey0 * P + lyn;
       const long idx = (long)gy * prm.pitch + gx;
       const double2 f = sf[lyn][lxn + lxn / P];
-      node_update_core<P, MODE_F64>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
-                                    pm[r][cidx], double2{0.0, 0.0});
+      if constexpr (FUSED)
+        node_update_core<P, MODE_F64>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
+                                      pm[r][cidx], double2{0.0, 0.0});
+      else
+        prm.f[idx] = f;
     }
   };
-  if constexpr (PF == 4) {
+  if constexpr (FUSED && PF == 4) {
 #pragma unroll
     for (int c0 = 0; c0 < NSL; c0 += CHUNK) {
 #pragma unroll
@@ -316,7 +321,7 @@ __device__ __forceinline__ void reg64_body(PRM &prm, int tile, int step, int las
 // term of the reference's loop itself (no cross-lane moves), in the same
 // order as the lane-pair body above.  A workgroup of EX x EY threads
 // processes one tile of (EX-1) x (EY-1) elements.
-template <int P, int EX, int EY, int PF, typename PRM>
+template <int P, int EX, int EY, int PF, bool FUSED = true, typename PRM>
 __device__ __forceinline__ void reg64s_body(PRM &prm, int tile, int step, int last) {
 #pragma clang fp contract(off)
   constexpr int NB = P + 1;
@@ -487,7 +492,7 @@ __device__ __forceinline__ void reg64s_body(PRM &prm, int tile, int step, int la
   __syncthreads();
   const TileLists tl = tile_lists(prm, tile, step, last);
   constexpr int NSL = RPW * CPL, CHUNK = 8;
-  if constexpr (PF == 0) {
+  if constexpr (FUSED && PF == 0) {
 #pragma unroll
     for (int r = 0; r < RPW; r++)
 #pragma unroll
@@ -528,11 +533,14 @@ __device__ __forceinline__ void reg64s_body(PRM &prm, int tile, int step, int la
       const int gx = ex0 * P + lxn, gy = ey0 * P + lyn;
       const long idx = (long)gy * prm.pitch + gx;
       const double2 f = sf[lyn][lxn + lxn / P];
-      node_update_core<P, MODE_F64>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
-                                    pm[r][cidx], double2{0.0, 0.0});
+      if constexpr (FUSED)
+        node_update_core<P, MODE_F64>(prm, tl, gx, gy, idx, f.x, f.y, pu[r][cidx].x, pu[r][cidx].y, pv[r][cidx],
+                                      pm[r][cidx], double2{0.0, 0.0});
+      else
+        prm.f[idx] = f;
     }
   };
-  if constexpr (PF == 4) {
+  if constexpr (FUSED && PF == 4) {
 #pragma unroll
     for (int c0 = 0; c0 < NSL; c0 += CHUNK) {
 #pragma unroll
@@ -555,6 +563,18 @@ void reg64_kernel(StepParams<double> prm) {
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
   if constexpr (LPE == 2) reg64_body<P, EX, EY, PF>(prm, tile, prm.step, prm.last);
   else reg64s_body<P, EX, EY, PF>(prm, tile, prm.step, prm.last);
+}
+
+// f = K(u) alone (the split schedule's element kernel and
+// se2w_assemble_forces): the same element operator and owner sums, the
+// nodal force stored instead of the update.
+template <int P, int EX, int EY, int WAVES, int PF, int LPE>
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+void reg64_element_kernel(StepParams<double> prm) {
+  const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
+  if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
+  if constexpr (LPE == 2) reg64_body<P, EX, EY, PF, false>(prm, tile, 0, 0);
+  else reg64s_body<P, EX, EY, PF, false>(prm, tile, 0, 0);
 }
 
 }  // namespace se2w

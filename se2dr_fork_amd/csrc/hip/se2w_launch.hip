@@ -74,6 +74,16 @@ void launch_element_p(const LaunchCfg &c, const StepParams<R> &prm) {
   if constexpr (kMode == MODE_F64 && FUSED) {
     if (c.variant >= SE2W_KERNEL_REG64 && launch_reg64_s<P, 0>(c, prm)) return;
   }
+  if constexpr (kMode == MODE_F64 && !FUSED) {
+    if constexpr (reg64_schedule_built(P, 0, false)) {
+      if (c.variant == SE2W_KERNEL_REG64) {
+        constexpr Reg64Schedule r = kReg64Schedules[0];
+        hipLaunchKernelGGL((reg64_element_kernel<P, r.ex, r.ey, r.waves, r.pf, r.lpe>), grid,
+                           dim3(r.ex * r.ey * r.lpe), 0, c.stream, prm);
+        return;
+      }
+    }
+  }
   if constexpr (FUSED) {
     if (c.variant == SE2W_KERNEL_LDS_WIDE) {
       hipLaunchKernelGGL((tile_kernel<P, kMode, true, 512>), grid, dim3(512), 0, c.stream, prm);
