@@ -179,7 +179,7 @@ int se2w_get_timing(se2w_ctx *ctx, double *run_ms, double kernel_ms[2]);
 #define SE2W_KERNEL_MFMA 9 /* FP32, 6 <= p <= 8, fused: element contractions on the matrix cores */
 #define SE2W_KERNEL_LINE 10 /* FP32, 6 <= p <= 8, fused: the same layout, contractions one line per thread */
 #define SE2W_KERNEL_LDS_WIDE 11 /* the LDS-staged tile kernel with 512 threads per workgroup (all modes, fused) */
-#define SE2W_KERNEL_REG64 12 /* FP64, 2 <= p <= 6, fused: register-blocked kernel in the reference's operation
+#define SE2W_KERNEL_REG64 12 /* FP64, 2 <= p <= 7, fused: register-blocked kernel in the reference's operation
                                 order, first schedule; SE2W_KERNEL_REG64 + k for the k-th (seven in all: a lane
                                 pair or one lane per element, several tile shapes; above p = 4 the lane pairs) */
 

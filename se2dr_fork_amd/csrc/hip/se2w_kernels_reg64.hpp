@@ -42,11 +42,12 @@ constexpr Reg64Schedule kReg64Schedules[] = {{32, 4, 3, 0, 2}, {32, 4, 0, 0, 2},
                                              {32, 4, 2, 0, 1}, {64, 2, 2, 0, 1}, {32, 4, 2, 4, 1}};
 constexpr int kNumReg64Schedules = sizeof(kReg64Schedules) / sizeof(kReg64Schedules[0]);
 // Degrees with a register-resident element (two (p+1)^2 arrays of doubles
-// per lane): 2 <= p <= 6; above p = 4 only the lane pairs (one lane per
-// element would need over 300 VGPRs at p = 5).  Every schedule for the fused
-// step, the first one also as the element kernel of the split schedule
-// (f = K(u) stored, no update).
-constexpr int kReg64MaxP = 6;
+// per lane): 2 <= p <= 7; above p = 4 only the lane pairs (one lane per
+// element would need over 300 VGPRs at p = 5).  At p = 8 the lane pair's
+// 162 doubles spill to scratch and the LDS tile kernel is faster.  Every
+// schedule for the fused step, the first one also as the element kernel of
+// the split schedule (f = K(u) stored, no update).
+constexpr int kReg64MaxP = 7;
 constexpr bool reg64_schedule_built(int P, int S, bool fused) {
   return P >= 2 && P <= kReg64MaxP && S >= 0 && S < kNumReg64Schedules && (P <= 4 || kReg64Schedules[S].lpe == 2) &&
          (fused || S == 0);

@@ -85,7 +85,7 @@ FP64_VARIANTS = [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in
 
 
 @pytest.mark.parametrize("p, mx, my", [(2, 9, 7), (3, 33, 5), (4, 41, 9), (4, 3, 2), (4, 70, 13), (4, 40, 1),
-                                        (4, 1, 17), (5, 23, 11), (6, 33, 9), (6, 2, 17)])
+                                        (4, 1, 17), (5, 23, 11), (6, 33, 9), (6, 2, 17), (7, 19, 7)])
 def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
     """Every FP64 step kernel (LDS tile kernels, register kernel schedules;
     above p = 4 the lane-pair schedules only) runs the reference's operation
