@@ -149,30 +149,33 @@ __device__ __forceinline__ void reg64_body(PRM &prm, int tile, int step, int las
       asm volatile("" : "+v"(ix));
 #pragma unroll
       for (int i = 0; i < NB; i++) dnx[i] = ix * D[qi * NB + i];
+      // e2 is summed by the x lane from its own column products of ux and
+      // the y lane's row products of uy (one broadcast each), then handed to
+      // the y lane; the y lane's running e2 is discarded.
       double srow = 0.0, scol = 0.0, e2 = 0.0;
 #pragma unroll
       for (int j = 0; j < qj; j++) {  // column products below the point's row
         const double t = dny[j] * U[j][qi];
         scol = scol + t;
-        e2 = e2 + from_x(t);
+        e2 = e2 + t;
       }
       const double tq = dny[qj] * U[qj][qi];
       scol = scol + tq;
-      const double tqx = from_x(tq);
 #pragma unroll
       for (int i = 0; i < NB; i++) {  // the point's row
         const double t = dnx[i] * U[qj][i];
         srow = srow + t;
         double r = from_y(t);
-        if (i == qi) r = r + tqx;
+        if (i == qi) r = r + tq;
         e2 = e2 + r;
       }
 #pragma unroll
       for (int j = qj + 1; j < NB; j++) {  // column products above the row
         const double t = dny[j] * U[j][qi];
         scol = scol + t;
-        e2 = e2 + from_x(t);
+        e2 = e2 + t;
       }
+      e2 = from_x(e2);
       // e0 = the x lane's row sum, e1 = the y lane's column sum (1372-1377)
       const double e0 = from_x(srow), e1 = from_y(scol);
       const double T1 = cA * e0 + cB * e1, T2 = mu * e2;
