@@ -281,10 +281,28 @@ def cpu_rate(sc, dt, threads, seconds_target, io_dir=None):
     return sc.nnodes * (n - 1) / el, n - 1, el
 
 
-def cpu_baseline(sc, dt, seconds_target=8.0):
+def config1_gpu(S, SC):
+    """The reference's own CPU-sized case (config 1: 64x64 elements, p = 6,
+    point force, 1000 steps at 0.2 CFL) in FP64 on the GPU: wall time of
+    the whole 1000-step run (setup excluded) and its receiver rows."""
+    sc = SC.config1()
+    b = SC.build(sc, precision=S.FP64)
+    b.solver.reserve(b.nsteps)
+    t0 = time.perf_counter()
+    _, tr = b.solver.advance(b.nsteps, b.dt)
+    t = time.perf_counter() - t0
+    variant = b.solver.ctx.kernel()
+    b.solver.close()
+    return {"sc": sc, "dt": b.dt, "steps": b.nsteps, "seconds": t, "trace": tr,
+            "kernel": kernel_signature(variant, sc.p, S.FP64)}
+
+
+def cpu_baseline(sc, dt, seconds_target=8.0, config1=None):
     """The reference algorithm on the host's cores: one core, and all the
     cores this process may use (at most 16, the box's share), compute only
-    and with the reference's per-step receiver file append."""
+    and with the reference's per-step receiver file append.  With config1
+    (the GPU's FP64 run of config 1), also the whole config-1 run on all
+    cores, and whether the GPU's receiver rows equal its bits."""
     import tempfile
 
     model, nproc, usable = cpu_info()
@@ -303,7 +321,24 @@ def cpu_baseline(sc, dt, seconds_target=8.0):
             "all_cores_with_receiver_io": {"value": allio, "threads": cores, "steps": nio, "seconds": elio,
                                            "what": "each step appends its 16-station row to the receiver file "
                                                    "(fopen/fprintf/fclose), as se2wave.c:2904-2981 does"},
-            "cpu_model": model, "nproc": nproc, "usable_cpus": usable}
+            "cpu_model": model, "nproc": nproc, "usable_cpus": usable,
+            "config1": config1_cpu(config1, cores) if config1 else None}
+
+
+def config1_cpu(c1, cores):
+    """Config 1 in full on the host (the oracle, all cores) against the GPU's
+    FP64 run of it."""
+    from oracle import oracle as O
+
+    sc = c1["sc"]
+    pr = O.Problem(sc, dt=c1["dt"], nt=c1["steps"], nthreads=cores)
+    t0 = time.perf_counter()
+    _, tr, _, _, _ = pr.run(want_state=False)
+    t = time.perf_counter() - t0
+    return {"workload": f"{sc.mx}x{sc.my} elements, p={sc.p}, point force, {c1['steps']} steps (the whole run)",
+            "gpu_fp64_s": c1["seconds"], "gpu_kernel": c1["kernel"], "cpu_fp64_s": t, "cpu_threads": cores,
+            "speedup": t / c1["seconds"],
+            "receiver_rows_bit_identical": bool(np.array_equal(tr, c1["trace"]))}
 
 
 def ensemble_bench(S, SC, n=256, nt=1500):
@@ -696,8 +731,10 @@ def main():
     if not args.no_surrogate:
         out["surrogate"] = surrogate_bench(S, SC)
     if not args.no_cpu:
+        log("config 1 on the GPU (FP64)")
+        c1 = config1_gpu(S, SC)
         log("host CPU baseline")
-        out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"])
+        out["cpu_baseline"] = cpu_baseline(sc, res[head]["dt"], config1=c1)
     else:
         out["cpu_baseline"] = None
     print(json.dumps(out))
