@@ -1292,7 +1292,7 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
   if (is_reg64_variant(variant) && !reg64_variant_built(c->p, c->mode, variant, c->schedule))
     return fail(SE2W_ERR_UNSUPPORTED,
                 "the FP64 register-blocked kernel needs FP64, 2 <= p <= %d (a lane pair per element above p = 4) "
-                "and the fused schedule",
+                "and the fused schedule (the split schedule: its first form only)",
                 kReg64MaxP);
   if (variant == SE2W_KERNEL_LDS_WIDE && c->schedule != SE2W_SCHED_FUSED)
     return fail(SE2W_ERR_UNSUPPORTED, "the 512-thread tile kernel is built for the fused schedule");
