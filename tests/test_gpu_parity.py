@@ -165,13 +165,20 @@ def test_readme_demo_figure():
     assert -6.3e-9 < vy[j] < -5.7e-9 and 0.182 < times[j] < 0.192
 
 
-def test_split_schedule_equals_fused(oracle):
-    sc = SC.small(mx=9, my=7, p=4, nt=30)
-    _, _, r1, u1, v1, a1 = gpu_run(sc, schedule=S.SCHED_FUSED)
+@pytest.mark.parametrize("p", [4, 6, 7])
+def test_split_schedule_equals_fused(oracle, p):
+    """The split schedule (element kernel storing f, then the update kernel;
+    in FP64 the register kernel's element-only form for 2 <= p <= 7) gives
+    the fused step's bits, and the oracle's."""
+    sc = SC.small(mx=9 + p, my=7, p=p, nt=30, seed=70 + p)
+    b, _, r1, u1, v1, a1 = gpu_run(sc, schedule=S.SCHED_FUSED)
     _, _, r2, u2, v2, a2 = gpu_run(sc, schedule=S.SCHED_SPLIT)
     np.testing.assert_array_equal(r1, r2)
     np.testing.assert_array_equal(u1, u2)
     np.testing.assert_array_equal(a1, a2)
+    _, ro, uo, _, _ = oracle_run(oracle, sc, b)
+    np.testing.assert_array_equal(r2, ro)
+    np.testing.assert_array_equal(u2, uo)
     for prec in [S.FP32, S.FP32_KAHAN]:
         _, _, r1, u1, _, _ = gpu_run(sc, precision=prec, schedule=S.SCHED_FUSED)
         _, _, r2, u2, _, _ = gpu_run(sc, precision=prec, schedule=S.SCHED_SPLIT)
