@@ -165,12 +165,15 @@ def kernel_name(variant, p, mode):
     return f"{kernel_signature(variant, p, mode)}: fused element operator + Verlet update, {what}"
 
 
-def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, kernel=None, reps=3):
-    """`reps` timed advances of `steps` steps each after `warmup` untimed
+def run_precision(S, SC, sc, prec, steps, warmup, full_steps, profile_steps, kernel=None, reps=None):
+    """`reps` timed advances of `steps` steps each (default 3, or 9 for runs
+    under 200 steps, whose few milliseconds see host jitter) after `warmup` untimed
     steps; the run buffers are sized for the longest advance beforehand, and
     the timed advances are checked to allocate nothing.  Returns the median
     wall time, the per-launch kernel time and (full_steps > 0) the receiver
     rows of a separate full-length run."""
+    if reps is None:
+        reps = 3 if steps >= 200 else 9
     b = SC.build(sc, precision=prec)
     sv = b.solver
     if kernel is not None:
