@@ -113,12 +113,22 @@ def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
     assert ran == len(FP64_VARIANTS) - (3 if p > 4 else 0)
 
 
-@pytest.mark.parametrize("name", ["small_p3", "small_p4_force", "small_p8"])
+def legacy_p4():
+    """The scenario of tests/golden/small_legacy_p4.npz: cubic-spline, P0 and
+    cubic-spline (h = 600) moment sources on 9x8 elements of degree 4."""
+    sc = SC.small(mx=9, my=8, p=4, nt=80, nsrc=3, nrec=6, seed=17)
+    sc.src_impl = [S.IMPL_SPLINE, S.IMPL_P0, S.IMPL_SPLINE]
+    sc.src_h = [0.0, 0.0, 600.0]
+    return sc
+
+
+@pytest.mark.parametrize("name", ["small_p3", "small_p4_force", "small_p8", "small_legacy_p4"])
 def test_fp64_golden_fixtures(name):
     z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
     sc = {"small_p3": lambda: SC.small(mx=6, my=5, p=3, nt=120),
           "small_p4_force": lambda: SC.small(mx=7, my=3, p=4, kind=S.SOURCE_FORCE, nt=100),
-          "small_p8": lambda: SC.small(mx=4, my=3, p=8, seed=5, nt=80)}[name]()
+          "small_p8": lambda: SC.small(mx=4, my=3, p=8, seed=5, nt=80),
+          "small_legacy_p4": legacy_p4}[name]()
     b, times, rows, u, v, a = gpu_run(sc)
     assert b.dt == float(z["dt"])
     np.testing.assert_array_equal(rows, z["trace"])

@@ -591,3 +591,33 @@ def test_oracle_reproduces_the_reference_figure():
     late = (t > 0.2) & (t < 0.35)
     assert 2.5e-9 < vy[late].max() < 3.5e-9 and -3.3e-9 < vy[late].min() < -2.4e-9
     assert t[-1] == pytest.approx(0.4, abs=z["dt"])
+
+
+@pytest.mark.parametrize("name", ["small_p3", "small_p4_force", "small_p8", "small_legacy_p4"])
+def test_oracle_reproduces_its_small_fixtures(oracle, name):
+    """The committed short-run fixtures (tests/golden/make_golden.py) are
+    reproduced bit for bit: any change to the oracle's arithmetic, including
+    the cubic-spline and P0 sources of small_legacy_p4, shows here."""
+    import os
+
+    import se2dr_fork_amd as S
+    from conftest import GOLDEN
+    from se2dr_fork_amd import scenarios as SC
+
+    z = np.load(os.path.join(GOLDEN, f"{name}.npz"))
+    if name == "small_legacy_p4":
+        sc = SC.small(mx=9, my=8, p=4, nt=80, nsrc=3, nrec=6, seed=17)
+        sc.src_impl = [S.IMPL_SPLINE, S.IMPL_P0, S.IMPL_SPLINE]
+        sc.src_h = [0.0, 0.0, 600.0]
+    else:
+        sc = {"small_p3": lambda: SC.small(mx=6, my=5, p=3, nt=120),
+              "small_p4_force": lambda: SC.small(mx=7, my=3, p=4, kind=S.SOURCE_FORCE, nt=100),
+              "small_p8": lambda: SC.small(mx=4, my=3, p=8, seed=5, nt=80)}[name]()
+    b = SC.build(sc, with_solver=False)
+    assert b.dt == float(z["dt"])
+    times, trace, u, v, a = oracle.Problem(sc, dt=b.dt, nt=b.nsteps).run()
+    np.testing.assert_array_equal(times, z["times"])
+    np.testing.assert_array_equal(trace, z["trace"])
+    np.testing.assert_array_equal(u, z["u"])
+    np.testing.assert_array_equal(v, z["v"])
+    np.testing.assert_array_equal(a, z["a"])
