@@ -59,6 +59,177 @@ __device__ __forceinline__ float pair_swap(float x) {
 // Component c (0 = x, 1 = y) of an LDS float2.
 __device__ __forceinline__ float &comp(float2 &v, int c) { return reinterpret_cast<float *>(&v)[c]; }
 
+// Loads element (ex, ey)'s nodal displacements, [j][i]: both components
+// (LPE = 1) or component c into ux (LPE = 2).
+template <int P, int LPE, typename PRM>
+__device__ __forceinline__ void reg_load_element(PRM &prm, int ex, int ey, int c, float (&ux)[P + 1][P + 1],
+                                                 float (&uy)[P + 1][P + 1]) {
+  constexpr int NB = P + 1;
+  const long base = (long)(ey * P) * prm.pitch + ex * P;
+  if constexpr (LPE == 1) {
+    const float2 *ub = prm.u_in + base;
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+        const float2 t = ub[(long)j * prm.pitch + i];
+        ux[j][i] = t.x;
+        uy[j][i] = t.y;
+      }
+    }
+  } else {
+    const float *ub = reinterpret_cast<const float *>(prm.u_in + base) + c;
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int i = 0; i < NB; i++) ux[j][i] = ub[2 * ((long)j * prm.pitch + i)];
+  }
+}
+
+// The element operator of one lane: the nodal forces of element (ex, ey)
+// from its displacements (LPE = 1: both components; LPE = 2: component c,
+// the partner lane holding the other).  sWq: the quadrature weights in LDS
+// (W_LDS, p >= 5) or unused.
+template <int P, int LPE, bool W_LDS>
+struct RegOperator {
+  static constexpr int NB = P + 1;
+  // D and the weights are read through the constant address space: uniform,
+  // never written by a kernel, so they are scalar loads into SGPRs also when
+  // prm itself is not a kernel argument (the batched kernels).
+  using CF = const __attribute__((address_space(4))) float;
+  CF *D;
+  CF *W;
+  const float *sWq;
+  // Jacobians, determinant and the internal-force sign folded into the
+  // material constants: A = iJ00*fac*sxx etc. with fac = -detJ*w_q.
+  float a1, a2, a3, b1, b2, b3;
+  float rA, rB, cA, cB;
+
+  template <typename PRM>
+  __device__ __forceinline__ RegOperator(PRM &prm, int ex, int ey, int c, const float *sw) : sWq(sw) {
+    const int e = ex + ey * prm.mx;
+    const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
+    const float iJ00 = 1.f / J00, iJ11 = 1.f / J11, detJ = J00 * J11;
+    const float lam = prm.lambda[e], mu = prm.mu[e], c11 = 2.f * mu + lam;
+    const float s = -detJ;
+    a1 = s * c11 * iJ00 * iJ00, a2 = s * lam * iJ00 * iJ11, a3 = s * c11 * iJ11 * iJ11;
+    b1 = s * mu * iJ11 * iJ11, b2 = s * mu * iJ00 * iJ11, b3 = s * mu * iJ00 * iJ00;
+    D = (CF *)(uintptr_t)prm.D;
+    W = (CF *)(uintptr_t)prm.w2;
+    rA = c ? b3 : a1, rB = c ? b2 : a2, cA = c ? a3 : b1, cB = c ? a2 : b2;
+  }
+
+  __device__ __forceinline__ float weight(int qi, int qj) const { return W_LDS ? sWq[qi + qj * NB] : W[qi + qj * NB]; }
+
+  // Fac-weighted stresses of quadrature row qj, pre-multiplied by the
+  // Jacobian factor of the test-function derivative they meet:
+  //   A  = w (a1 dx ux + a2 dy uy)  iJ00 fac sxx  (row terms of fx)
+  //   B  = w (b1 dy ux + b2 dx uy)  iJ11 fac sxy  (column terms of fx)
+  //   Cx = w (b3 dx uy + b2 dy ux)  iJ00 fac sxy  (row terms of fy)
+  //   E  = w (a3 dy uy + a2 dx ux)  iJ11 fac syy  (column terms of fy)
+  // The first product of each is the one of the lane's own component, so
+  // LPE = 2 forms R (A or Cx) and Cq (B or E) from its own derivatives and
+  // the partner's with the same operations: R = w (rA own_dx + rB other_dy),
+  // Cq = w (cA own_dy + cB other_dx).
+  __device__ __forceinline__ void row_stress(int qj, const float (&ux)[NB][NB], const float (&uy)[NB][NB], float *A,
+                                             float *B, float *Cx, float *E) const {
+#pragma unroll
+    for (int qi = 0; qi < NB; qi++) {
+      float dxux = 0.f, dxuy = 0.f, dyux = 0.f, dyuy = 0.f;
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        dxux = fmaf(D[qi * NB + k], ux[qj][k], dxux);
+        dxuy = fmaf(D[qi * NB + k], uy[qj][k], dxuy);
+        dyux = fmaf(D[qj * NB + k], ux[k][qi], dyux);
+        dyuy = fmaf(D[qj * NB + k], uy[k][qi], dyuy);
+      }
+      const float w = weight(qi, qj);
+      A[qi] = w * fmaf(a1, dxux, a2 * dyuy);
+      E[qi] = w * fmaf(a3, dyuy, a2 * dxux);
+      B[qi] = w * fmaf(b1, dyux, b2 * dxuy);
+      Cx[qi] = w * fmaf(b3, dxuy, b2 * dyux);
+    }
+  }
+  __device__ __forceinline__ void row_stress_pair(int qj, const float (&ux)[NB][NB], float *R, float *Cq) const {
+#pragma unroll
+    for (int qi = 0; qi < NB; qi++) {
+      float dx = 0.f, dy = 0.f;
+#pragma unroll
+      for (int k = 0; k < NB; k++) {
+        dx = fmaf(D[qi * NB + k], ux[qj][k], dx);
+        dy = fmaf(D[qj * NB + k], ux[k][qi], dy);
+      }
+      const float odx = pair_swap(dx), ody = pair_swap(dy);
+      const float w = weight(qi, qj);
+      R[qi] = w * fmaf(rA, dx, rB * ody);
+      Cq[qi] = w * fmaf(cA, dy, cB * odx);
+    }
+  }
+
+  // All nodal forces, [j][i] (LPE = 2: fx only).
+  __device__ __forceinline__ void forces(const float (&ux)[NB][NB], const float (&uy)[NB][NB], float (&fx)[NB][NB],
+                                         float (&fy)[NB][NB]) const {
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+#pragma unroll
+      for (int i = 0; i < NB; i++) fx[j][i] = fy[j][i] = 0.f;
+#pragma unroll
+    for (int qj = 0; qj < NB; qj++) {
+      float A[NB], B[NB], Cx[NB], E[NB];
+      if constexpr (LPE == 1) row_stress(qj, ux, uy, A, B, Cx, E);
+      else row_stress_pair(qj, ux, A, B);
+      // row terms: nodes (i, qj) from the points of row qj
+#pragma unroll
+      for (int i = 0; i < NB; i++) {
+#pragma unroll
+        for (int qi = 0; qi < NB; qi++) {
+          fx[qj][i] = fmaf(D[qi * NB + i], A[qi], fx[qj][i]);
+          if constexpr (LPE == 1) fy[qj][i] = fmaf(D[qi * NB + i], Cx[qi], fy[qj][i]);
+        }
+      }
+      // column terms: nodes (qi, j) from the point (qi, qj)
+#pragma unroll
+      for (int j = 0; j < NB; j++) {
+#pragma unroll
+        for (int qi = 0; qi < NB; qi++) {
+          fx[j][qi] = fmaf(D[qj * NB + j], B[qi], fx[j][qi]);
+          if constexpr (LPE == 1) fy[j][qi] = fmaf(D[qj * NB + j], E[qi], fy[j][qi]);
+        }
+      }
+    }
+  }
+
+  // Only the top edge (j = p) of the forces, with the same operations in the
+  // same order as forces(), so its values are identical to forces()'s row p.
+  __device__ __forceinline__ void top_edge(const float (&ux)[NB][NB], const float (&uy)[NB][NB], float (&tx)[NB],
+                                           float (&ty)[NB]) const {
+#pragma unroll
+    for (int i = 0; i < NB; i++) tx[i] = ty[i] = 0.f;
+#pragma unroll
+    for (int qj = 0; qj < NB; qj++) {
+      float A[NB], B[NB], Cx[NB], E[NB];
+      if constexpr (LPE == 1) row_stress(qj, ux, uy, A, B, Cx, E);
+      else row_stress_pair(qj, ux, A, B);
+      if (qj == P) {
+#pragma unroll
+        for (int i = 0; i < NB; i++) {
+#pragma unroll
+          for (int qi = 0; qi < NB; qi++) {
+            tx[i] = fmaf(D[qi * NB + i], A[qi], tx[i]);
+            if constexpr (LPE == 1) ty[i] = fmaf(D[qi * NB + i], Cx[qi], ty[i]);
+          }
+        }
+      }
+#pragma unroll
+      for (int qi = 0; qi < NB; qi++) {
+        tx[qi] = fmaf(D[qj * NB + P], B[qi], tx[qi]);
+        if constexpr (LPE == 1) ty[qi] = fmaf(D[qj * NB + P], E[qi], ty[qi]);
+      }
+    }
+  }
+};
+
+
 // One workgroup of EX x EY x LPE threads processes one tile of (EX-1) x
 // (EY-1) elements (all threads call it with the same tile).  A workgroup that
 // calls it for several tiles in a row puts a barrier between the calls.
@@ -141,112 +312,13 @@ __device__ __forceinline__ void reg_body(PRM &prm, int tile, int step, int last)
   float ux[NB][NB], uy[NB][NB];  // [j][i]
   float fx[NB][NB], fy[NB][NB];
   if (valid) {
-    const long base = (long)(ey * P) * prm.pitch + ex * P;
-    if constexpr (LPE == 1) {
-      const float2 *ub = prm.u_in + base;
-#pragma unroll
-      for (int j = 0; j < NB; j++) {
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-          const float2 t = ub[(long)j * prm.pitch + i];
-          ux[j][i] = t.x;
-          uy[j][i] = t.y;
-        }
-      }
-    } else {
-      const float *ub = reinterpret_cast<const float *>(prm.u_in + base) + c;
-#pragma unroll
-      for (int j = 0; j < NB; j++)
-#pragma unroll
-        for (int i = 0; i < NB; i++) ux[j][i] = ub[2 * ((long)j * prm.pitch + i)];
-    }
-    const int e = ex + ey * prm.mx;
-    const float J00 = 0.5f * prm.elem_dx[ex], J11 = 0.5f * prm.elem_dy[ey];
-    const float iJ00 = 1.f / J00, iJ11 = 1.f / J11, detJ = J00 * J11;
-    const float lam = prm.lambda[e], mu = prm.mu[e], c11 = 2.f * mu + lam;
-    // Jacobians, determinant and the internal-force sign folded into the
-    // material constants: A = iJ00*fac*sxx etc. with fac = -detJ*w_q.
-    const float s = -detJ;
-    const float a1 = s * c11 * iJ00 * iJ00, a2 = s * lam * iJ00 * iJ11, a3 = s * c11 * iJ11 * iJ11;
-    const float b1 = s * mu * iJ11 * iJ11, b2 = s * mu * iJ00 * iJ11, b3 = s * mu * iJ00 * iJ00;
-    // D and the weights are read through the constant address space: uniform,
-    // never written by a kernel, so they are scalar loads into SGPRs also when
-    // prm itself is not a kernel argument (the batched kernels).
-    using CF = const __attribute__((address_space(4))) float;
-    CF *D = (CF *)(uintptr_t)prm.D;
-    CF *W = (CF *)(uintptr_t)prm.w2;
-    // Fac-weighted stresses of quadrature row qj, pre-multiplied by the
-    // Jacobian factor of the test-function derivative they meet:
-    //   A  = w (a1 dx ux + a2 dy uy)  iJ00 fac sxx  (row terms of fx)
-    //   B  = w (b1 dy ux + b2 dx uy)  iJ11 fac sxy  (column terms of fx)
-    //   Cx = w (b3 dx uy + b2 dy ux)  iJ00 fac sxy  (row terms of fy)
-    //   E  = w (a3 dy uy + a2 dx ux)  iJ11 fac syy  (column terms of fy)
-    // The first product of each is the one of the lane's own component, so
-    // LPE = 2 forms R (A or Cx) and Cq (B or E) from its own derivatives and
-    // the partner's with the same operations: R = w (rA own_dx + rB other_dy),
-    // Cq = w (cA own_dy + cB other_dx).
-    auto row_stress = [&](int qj, float *A, float *B, float *Cx, float *E) {
-#pragma unroll
-      for (int qi = 0; qi < NB; qi++) {
-        float dxux = 0.f, dxuy = 0.f, dyux = 0.f, dyuy = 0.f;
-#pragma unroll
-        for (int k = 0; k < NB; k++) {
-          dxux = fmaf(D[qi * NB + k], ux[qj][k], dxux);
-          dxuy = fmaf(D[qi * NB + k], uy[qj][k], dxuy);
-          dyux = fmaf(D[qj * NB + k], ux[k][qi], dyux);
-          dyuy = fmaf(D[qj * NB + k], uy[k][qi], dyuy);
-        }
-        const float w = W_LDS ? sWq[qi + qj * NB] : W[qi + qj * NB];
-        A[qi] = w * fmaf(a1, dxux, a2 * dyuy);
-        E[qi] = w * fmaf(a3, dyuy, a2 * dxux);
-        B[qi] = w * fmaf(b1, dyux, b2 * dxuy);
-        Cx[qi] = w * fmaf(b3, dxuy, b2 * dyux);
-      }
-    };
-    const float rA = c ? b3 : a1, rB = c ? b2 : a2, cA = c ? a3 : b1, cB = c ? a2 : b2;
-    auto row_stress_pair = [&](int qj, float *R, float *Cq) {
-#pragma unroll
-      for (int qi = 0; qi < NB; qi++) {
-        float dx = 0.f, dy = 0.f;
-#pragma unroll
-        for (int k = 0; k < NB; k++) {
-          dx = fmaf(D[qi * NB + k], ux[qj][k], dx);
-          dy = fmaf(D[qj * NB + k], ux[k][qi], dy);
-        }
-        const float odx = pair_swap(dx), ody = pair_swap(dy);
-        const float w = W_LDS ? sWq[qi + qj * NB] : W[qi + qj * NB];
-        R[qi] = w * fmaf(rA, dx, rB * ody);
-        Cq[qi] = w * fmaf(cA, dy, cB * odx);
-      }
-    };
+    reg_load_element<P, LPE>(prm, ex, ey, c, ux, uy);
+    const RegOperator<P, LPE, W_LDS> op(prm, ex, ey, c, sWq);
     if (HALO_WAVES && ly == 0) {
       // Halo row: only the top edge (j = p; node (p, p) is also the corner
-      // the owners read from redge), with the same operations in the same
-      // order as the full path below, so the edge values are identical to it.
+      // the owners read from redge).
       float tx[NB], ty[NB];
-#pragma unroll
-      for (int i = 0; i < NB; i++) tx[i] = ty[i] = 0.f;
-#pragma unroll
-      for (int qj = 0; qj < NB; qj++) {
-        float A[NB], B[NB], Cx[NB], E[NB];
-        if constexpr (LPE == 1) row_stress(qj, A, B, Cx, E);
-        else row_stress_pair(qj, A, B);
-        if (qj == P) {
-#pragma unroll
-          for (int i = 0; i < NB; i++) {
-#pragma unroll
-            for (int qi = 0; qi < NB; qi++) {
-              tx[i] = fmaf(D[qi * NB + i], A[qi], tx[i]);
-              if constexpr (LPE == 1) ty[i] = fmaf(D[qi * NB + i], Cx[qi], ty[i]);
-            }
-          }
-        }
-#pragma unroll
-        for (int qi = 0; qi < NB; qi++) {
-          tx[qi] = fmaf(D[qj * NB + P], B[qi], tx[qi]);
-          if constexpr (LPE == 1) ty[qi] = fmaf(D[qj * NB + P], E[qi], ty[qi]);
-        }
-      }
+      op.top_edge(ux, uy, tx, ty);
       if constexpr (LPE == 1) {
 #pragma unroll
         for (int k = 0; k < NB; k++) tedge[ly][lx][k] = float2{tx[k], ty[k]};
@@ -257,34 +329,7 @@ __device__ __forceinline__ void reg_body(PRM &prm, int tile, int step, int last)
         comp(redge[ly][lx][P], c) = tx[P];
       }
     } else {
-#pragma unroll
-      for (int j = 0; j < NB; j++)
-#pragma unroll
-        for (int i = 0; i < NB; i++) fx[j][i] = fy[j][i] = 0.f;
-#pragma unroll
-      for (int qj = 0; qj < NB; qj++) {
-        float A[NB], B[NB], Cx[NB], E[NB];
-        if constexpr (LPE == 1) row_stress(qj, A, B, Cx, E);
-        else row_stress_pair(qj, A, B);
-        // row terms: nodes (i, qj) from the points of row qj
-#pragma unroll
-        for (int i = 0; i < NB; i++) {
-#pragma unroll
-          for (int qi = 0; qi < NB; qi++) {
-            fx[qj][i] = fmaf(D[qi * NB + i], A[qi], fx[qj][i]);
-            if constexpr (LPE == 1) fy[qj][i] = fmaf(D[qi * NB + i], Cx[qi], fy[qj][i]);
-          }
-        }
-        // column terms: nodes (qi, j) from the point (qi, qj)
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-#pragma unroll
-          for (int qi = 0; qi < NB; qi++) {
-            fx[j][qi] = fmaf(D[qj * NB + j], B[qi], fx[j][qi]);
-            if constexpr (LPE == 1) fy[j][qi] = fmaf(D[qj * NB + j], E[qi], fy[j][qi]);
-          }
-        }
-      }
+      op.forces(ux, uy, fx, fy);
       // publish the right and top edges for the neighbouring owners
 #pragma unroll
       for (int k = 0; k < NB; k++) {

@@ -358,7 +358,7 @@ def test_assemble_forces_with_every_kernel_variant(oracle, p, mx, my):
 
 
 @pytest.mark.parametrize("p, mx, my", [(4, 37, 33), (6, 23, 19), (4, 3, 2), (4, 40, 1), (2, 9, 7), (3, 8, 5),
-                                        (5, 11, 6), (7, 12, 9), (8, 17, 10)])
+                                        (5, 11, 6), (7, 12, 9), (8, 17, 10), (3, 70, 20), (6, 35, 34)])
 def test_fp32_kernel_variants_agree(p, mx, my):
     """Register-blocked schedules share the arithmetic (one lane or a lane pair
     per element, full or top-edge-only halo rows): identical bits.  The
