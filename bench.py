@@ -116,7 +116,7 @@ REG_SCHEDULES = [(32, 4, 0, 4, 1), (64, 2, 0, 4, 1), (32, 2, 0, 2, 1), (32, 4, 0
 # lanes per element) of the FP64 register kernel's schedules
 # (SE2W_KERNEL_REG64 + k)
 REG64_SCHEDULES = [(32, 4, 3, 0, 2), (32, 4, 0, 0, 2), (16, 8, 0, 4, 2), (32, 4, 3, 4, 2), (32, 4, 2, 0, 1),
-                   (64, 2, 2, 0, 1), (32, 4, 2, 4, 1)]
+                   (64, 2, 2, 0, 1), (32, 4, 2, 4, 1), (16, 4, 1, 4, 2), (32, 4, 1, 4, 2)]
 KERNEL_REG64 = 12
 
 

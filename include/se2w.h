@@ -179,9 +179,10 @@ int se2w_get_timing(se2w_ctx *ctx, double *run_ms, double kernel_ms[2]);
 #define SE2W_KERNEL_MFMA 9 /* FP32, 6 <= p <= 8, fused: element contractions on the matrix cores */
 #define SE2W_KERNEL_LINE 10 /* FP32, 6 <= p <= 8, fused: the same layout, contractions one line per thread */
 #define SE2W_KERNEL_LDS_WIDE 11 /* the LDS-staged tile kernel with 512 threads per workgroup (all modes, fused) */
-#define SE2W_KERNEL_REG64 12 /* FP64, 2 <= p <= 7, fused: register-blocked kernel in the reference's operation
-                                order, first schedule; SE2W_KERNEL_REG64 + k for the k-th (seven in all: a lane
-                                pair or one lane per element, several tile shapes; above p = 4 the lane pairs) */
+#define SE2W_KERNEL_REG64 12 /* FP64, 2 <= p <= 8, fused: register-blocked kernel in the reference's operation
+                                order, first schedule; SE2W_KERNEL_REG64 + k for the k-th (nine in all: a lane
+                                pair or one lane per element, several tile shapes; above p = 4 the lane pairs;
+                                k = 0..6 up to p = 7, k = 7, 8 at p = 8, one wave per SIMD) */
 
 /* Selects / reports the element-kernel variant (no counterpart in the
  * reference: a schedule choice of this implementation, used by the

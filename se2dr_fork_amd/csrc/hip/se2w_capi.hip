@@ -636,7 +636,9 @@ int se2w_create(const se2w_desc *desc, se2w_ctx **out) {
   c->variant = reg_available(c->p, c->mode) && reg_variant_built(c->p, SE2W_KERNEL_REG, c->schedule)
                    ? SE2W_KERNEL_REG
                : reg64_variant_built(c->p, c->mode, SE2W_KERNEL_REG64, c->schedule) ? SE2W_KERNEL_REG64
-                                                                                    : SE2W_KERNEL_LDS;
+               : reg64_variant_built(c->p, c->mode, SE2W_KERNEL_REG64 + kReg64FirstP8, c->schedule)
+                   ? SE2W_KERNEL_REG64 + kReg64FirstP8
+                   : SE2W_KERNEL_LDS;
   set_tiling(c);
   int ierr = SE2W_OK;
   hipError_t he = hipSetDevice(c->device);
@@ -1291,9 +1293,10 @@ int se2w_set_kernel(se2w_ctx *c, int32_t variant) {
     return fail(SE2W_ERR_ARG, "unknown kernel %d", variant);
   if (is_reg64_variant(variant) && !reg64_variant_built(c->p, c->mode, variant, c->schedule))
     return fail(SE2W_ERR_UNSUPPORTED,
-                "the FP64 register-blocked kernel needs FP64, 2 <= p <= %d (a lane pair per element above p = 4) "
-                "and the fused schedule (the split schedule: its first form only)",
-                kReg64MaxP);
+                "the FP64 register-blocked kernel needs FP64, 2 <= p <= %d (a lane pair per element above p = 4; "
+                "schedules %d and %d at p = 8, the others up to p = 7) and the fused schedule (the split "
+                "schedule: its first form only)",
+                kReg64MaxP, SE2W_KERNEL_REG64 + kReg64FirstP8, SE2W_KERNEL_REG64 + kNumReg64Schedules - 1);
   if (variant == SE2W_KERNEL_LDS_WIDE && c->schedule != SE2W_SCHED_FUSED)
     return fail(SE2W_ERR_UNSUPPORTED, "the 512-thread tile kernel is built for the fused schedule");
   if (variant == SE2W_KERNEL_MFMA && !mfma_available(c->p, c->mode, c->schedule))

@@ -62,7 +62,7 @@ void ens_reg_kernel(EnsArgs<float> a) {
 }
 
 template <int P, int EX, int EY, int WAVES, int PF, int LPE>
-__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES == 1 ? 1 : 8)))
 void ens_reg64_kernel(EnsArgs<double> a) {
   ens_tile(a, [](auto &prm, int tile, int step, int last) {
     if constexpr (LPE == 2) reg64_body<P, EX, EY, PF>(prm, tile, step, last);

@@ -32,25 +32,32 @@
 namespace se2w {
 
 // Schedules of the FP64 register kernel: workgroup shape (EX x EY elements,
-// two lanes each), a minimum number of waves per SIMD (0 = the compiler's
-// choice; 3 caps the registers at 168 per lane, which at p = 4 spills a few
-// to scratch) and the placement of the node-phase loads (0: after the force
+// two lanes each), waves per SIMD (0 = the compiler's choice; 3 caps the
+// registers at 168 per lane, which at p = 4 spills a few to scratch; 1 gives
+// a lane the whole register file, 256 VGPRs and 256 AGPRs, the AGPRs taking
+// the spills) and the placement of the node-phase loads (0: after the force
 // exchange barrier, all slots at once; 4: in the node phase, CHUNK slots at
 // a time).
 struct Reg64Schedule { int ex, ey, waves, pf, lpe; };
 constexpr Reg64Schedule kReg64Schedules[] = {{32, 4, 3, 0, 2}, {32, 4, 0, 0, 2}, {16, 8, 0, 4, 2}, {32, 4, 3, 4, 2},
-                                             {32, 4, 2, 0, 1}, {64, 2, 2, 0, 1}, {32, 4, 2, 4, 1}};
+                                             {32, 4, 2, 0, 1}, {64, 2, 2, 0, 1}, {32, 4, 2, 4, 1}, {16, 4, 1, 4, 2},
+                                             {32, 4, 1, 4, 2}};
 constexpr int kNumReg64Schedules = sizeof(kReg64Schedules) / sizeof(kReg64Schedules[0]);
+// The first schedule of the p = 8 pair.
+constexpr int kReg64FirstP8 = 7;
 // Degrees with a register-resident element (two (p+1)^2 arrays of doubles
-// per lane): 2 <= p <= 7; above p = 4 only the lane pairs (one lane per
-// element would need over 300 VGPRs at p = 5).  At p = 8 the lane pair's
-// 162 doubles spill to scratch and the LDS tile kernel is faster.  Every
-// schedule for the fused step, the first one also as the element kernel of
-// the split schedule (f = K(u) stored, no update).
-constexpr int kReg64MaxP = 7;
+// per lane): 2 <= p <= 8; above p = 4 only the lane pairs (one lane per
+// element would need over 300 VGPRs at p = 5).  The first seven schedules
+// for 2 <= p <= 7; at p = 8, where the lane pair's 2 x 162 VGPRs of nodal
+// values and forces exceed 256, the two one-wave schedules, which keep the
+// spills in AGPRs (48 bytes per lane go to scratch; the compiler's choice of
+// registers sent kilobytes there and ran slower than the LDS tile kernel).
+// Every schedule for the fused step, the first one also as the element
+// kernel of the split schedule (f = K(u) stored, no update).
+constexpr int kReg64MaxP = 8;
 constexpr bool reg64_schedule_built(int P, int S, bool fused) {
   return P >= 2 && P <= kReg64MaxP && S >= 0 && S < kNumReg64Schedules && (P <= 4 || kReg64Schedules[S].lpe == 2) &&
-         (fused || S == 0);
+         (S < kReg64FirstP8 ? P <= 7 && (fused || S == 0) : P == 8 && fused);
 }
 
 // The value that lane 2k (x) or lane 2k+1 (y) of a pair holds, in both
@@ -561,7 +568,7 @@ __device__ __forceinline__ void reg64s_body(PRM &prm, int tile, int step, int la
 }
 
 template <int P, int EX, int EY, int WAVES, int PF, int LPE>
-__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES == 1 ? 1 : 8)))
 void reg64_kernel(StepParams<double> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup
@@ -573,7 +580,7 @@ void reg64_kernel(StepParams<double> prm) {
 // se2w_assemble_forces): the same element operator and owner sums, the
 // nodal force stored instead of the update.
 template <int P, int EX, int EY, int WAVES, int PF, int LPE>
-__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, 8)))
+__global__ __launch_bounds__(EX * EY * LPE) __attribute__((amdgpu_waves_per_eu(WAVES > 0 ? WAVES : 1, WAVES == 1 ? 1 : 8)))
 void reg64_element_kernel(StepParams<double> prm) {
   const int tile = tile_of_block(blockIdx.x, prm.tiles_per_xcd);
   if (tile >= prm.tiles_x * prm.tiles_y) return;  // uniform per workgroup

@@ -77,7 +77,7 @@ def test_fp64_ensemble_bit_identical_to_oracle(oracle, p):
     ens.close()
 
 
-@pytest.mark.parametrize("p", [4, 6])
+@pytest.mark.parametrize("p", [4, 6, 8])
 def test_fp64_ensemble_every_variant_bit_identical_to_oracle(oracle, p):
     """Every FP64 batched kernel (the LDS tile kernels and the register
     kernel schedules built for p) advances each member exactly as the
@@ -89,18 +89,20 @@ def test_fp64_ensemble_every_variant_bit_identical_to_oracle(oracle, p):
         _, tr0, u0, v0, a0 = oracle.Problem(sc, dt=b.dt, nt=20).run(max_steps=20)
         want.append((tr0, u0, v0, a0))
     tried = 0
-    for var in [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in range(7)]:
+    for var in [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in range(9)]:
         builts, ens = ensemble(scs, S.FP64)
         try:
             ens.set_kernel(var)
         except S.Se2wError:
             ens.close()
-            assert var >= S.KERNEL_REG64 + 4 and p > 4  # one lane per element: p <= 4
+            # one lane per element: p <= 4; schedules 7, 8 only (and only they) at p = 8
+            s = var - S.KERNEL_REG64
+            assert (s >= 4 and p > 4) or ((s >= 7) != (p == 8))
             continue
         tried += 1
         assert_same(ens_runs(builts, ens, 20, chunks=(7, 13)), want)
         ens.close()
-    assert tried == (9 if p <= 4 else 6)
+    assert tried == {4: 9, 6: 6, 8: 4}[p]
 
 
 def test_fp64_ensemble_with_cubic_spline_and_p0_sources(oracle):

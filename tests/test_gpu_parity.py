@@ -81,17 +81,27 @@ def test_fp64_bit_identical_to_oracle(oracle, sc):
     np.testing.assert_array_equal(a, ao)
 
 
-FP64_VARIANTS = [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in range(7)]
+FP64_VARIANTS = [S.KERNEL_LDS, S.KERNEL_LDS_WIDE] + [S.KERNEL_REG64 + k for k in range(9)]
+
+
+def fp64_variant_built(k, p):
+    """Register schedules: 0..6 up to p = 7 (one lane per element, 4..6, up to
+    p = 4), 7 and 8 (one wave per SIMD) at p = 8; the LDS kernels always."""
+    if k < S.KERNEL_REG64:
+        return True
+    s = k - S.KERNEL_REG64
+    return (s >= 7) == (p == 8) and (s < 4 or s >= 7 or p <= 4)
 
 
 @pytest.mark.parametrize("p, mx, my", [(2, 9, 7), (3, 33, 5), (4, 41, 9), (4, 3, 2), (4, 70, 13), (4, 40, 1),
-                                        (4, 1, 17), (5, 23, 11), (6, 33, 9), (6, 2, 17), (7, 19, 7)])
+                                        (4, 1, 17), (5, 23, 11), (6, 33, 9), (6, 2, 17), (7, 19, 7), (8, 35, 9),
+                                        (8, 2, 5)])
 def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
     """Every FP64 step kernel (LDS tile kernels, register kernel schedules;
-    above p = 4 the lane-pair schedules only) runs the reference's operation
-    order: receiver rows and the whole state bit-identical to the oracle, on
-    meshes of one tile, several tiles with partial edge tiles, one element
-    row and one element column."""
+    above p = 4 the lane-pair schedules only, at p = 8 the one-wave ones) runs
+    the reference's operation order: receiver rows and the whole state
+    bit-identical to the oracle, on meshes of one tile, several tiles with
+    partial edge tiles, one element row and one element column."""
     sc = SC.small(mx=mx, my=my, p=p, nt=30, seed=100 + p + mx, nsrc=2, nrec=5)
     b0 = SC.build(sc)
     pr = oracle.Problem(sc, dt=b0.dt, nt=b0.nsteps)
@@ -102,15 +112,16 @@ def test_fp64_kernel_variants_bit_identical_to_oracle(oracle, p, mx, my):
         try:
             b, times, rows, u, v, a = gpu_run(sc, kernel=k, chunks=2)
         except S.Se2wError:
-            assert k >= S.KERNEL_REG64 + 4 and p > 4  # one lane per element: p <= 4
+            assert not fp64_variant_built(k, p)
             continue
+        assert fp64_variant_built(k, p)
         ran += 1
         np.testing.assert_array_equal(rows, ro, err_msg=f"variant {k}")
         np.testing.assert_array_equal(u, uo, err_msg=f"variant {k}")
         np.testing.assert_array_equal(v, vo, err_msg=f"variant {k}")
         np.testing.assert_array_equal(a, ao, err_msg=f"variant {k}")
         b.solver.close()
-    assert ran == len(FP64_VARIANTS) - (3 if p > 4 else 0)
+    assert ran == sum(fp64_variant_built(k, p) for k in FP64_VARIANTS)
 
 
 def legacy_p4():
