@@ -614,6 +614,17 @@ def test_gpu_error_paths():
         S.Context(4, 4, 9)
 
 
+def test_fp64_default_kernel_per_degree():
+    """A new FP64 context runs the FP64 register kernel wherever it is built:
+    its first schedule for 2 <= p <= 7, the first one-wave schedule at p = 8;
+    the LDS tile kernel at p = 1.  The p = 8 error names the schedules."""
+    for p in range(1, 9):
+        k = S.Context(6, 5, p, precision=S.FP64).kernel()
+        assert k == (S.KERNEL_LDS if p == 1 else S.KERNEL_REG64 + 7 if p == 8 else S.KERNEL_REG64), (p, k)
+    with pytest.raises(S.Se2wError, match="schedules 19 and 20 at p = 8"):
+        S.Context(6, 5, 8, precision=S.FP64).set_kernel(S.KERNEL_REG64)
+
+
 @pytest.mark.parametrize("p", [6, 7, 8])
 def test_mfma_kernel_matches_the_vector_kernels(p):
     """Matrix-core contractions (FP32 in, FP32 accumulate) sum in another
